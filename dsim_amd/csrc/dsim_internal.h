@@ -1,0 +1,91 @@
+// Internal types shared by the host runtime (host.cpp) and the HIP kernels
+// (kernels.hip).  Plain structs only: the kernels take them by value.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+
+namespace dsim {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kFiberNone = 65535u;
+
+// One 64-byte record per triangle for each collision test.
+//   reference record: h0..h11 of spinSystem.cpp:612-648, then the first
+//                     vertex P1 (x,y,z) and info (fibre | membrane<<16)
+//   MT record:        n.x n.y n.z n.v0 | v0.x v0.y v0.z info | e1 - | e2 -
+// Labelling uses 32-byte AABB records: min.xyz info | max.xyz 0.
+struct StepParams {
+    // substrate (device pointers)
+    const float4 *tri_ref;
+    const float4 *tri_mt;
+    const uint32_t *cell_start;
+    const uint32_t *cell_tri;
+    const uint8_t *fiber_myelin;
+    uint32_t n_fibers;
+    uint32_t dims[3];
+    float lo[3];
+    float cell[3];
+    float inv_cell[3];
+    // physics
+    float std_dev[4];
+    float t2[4];
+    float decay[4];
+    float dt;
+    float perm;
+    uint32_t reflection_type;
+    float clamp_hi, clamp_lo;
+    // spin state (device pointers, SoA)
+    uint32_t n;
+    float *x, *y, *z, *mag, *phase;
+    uint32_t *label;  // compartment | fibre << 8
+    uint32_t *seed;   // MWC, 2 per spin
+    float *disp;      // [3][n] unwrapped displacement
+    float *mom;       // [3][n] moment
+    float *momc;      // [3][n] Kahan compensation
+    // per call
+    float gx, gy, gz, pc, weight;
+    uint32_t key0, key1;
+    uint64_t step0;
+    const float4 *inject;  // [iters][n]
+    uint32_t *trace;       // [iters][n][5]
+    uint32_t *flags;       // device status word (bit 0: collision loop guard hit)
+};
+
+struct LabelParams {
+    const float4 *tri_ref;
+    const float4 *tri_aabb;
+    uint32_t n_tri;
+    uint32_t n;
+    const float *x, *y, *z;
+    uint32_t *label;
+    uint32_t *tries;
+    float xmax, ymax, zmax;
+    uint32_t max_tries;
+    uint32_t plus_x;  // 1: ray along +x, 0: dominant axis (reference)
+    uint32_t *flags;  // bit 1: two odd fibres on one membrane type
+};
+
+enum Rng { kRngInject = 0, kRngMwc = 1, kRngPhilox = 2 };
+
+// Launchers (kernels.hip).  Return hipError_t of the launch.
+hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
+                       bool trace, bool moment, hipStream_t s);
+hipError_t launch_signal(uint32_t n, const float *mag, const float *phase, const uint32_t *label,
+                         double *partial, uint32_t n_blocks, double *out15, hipStream_t s);
+hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, const float *x0,
+                                float wsum, const uint32_t *label, uint32_t ndir,
+                                const float *grads, float pc, double *partial, uint32_t n_blocks,
+                                double *out, hipStream_t s);
+hipError_t launch_label(const LabelParams &p, hipStream_t s);
+hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
+                               uint32_t *label, float start_box, uint32_t key0, uint32_t key1,
+                               hipStream_t s);
+hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
+                               float *x0, float *disp, float *mom, float *momc, hipStream_t s);
+
+constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout kernels
+
+}  // namespace dsim

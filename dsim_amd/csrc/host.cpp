@@ -1,0 +1,934 @@
+// Host runtime of the dsim C ABI (include/dsim.h): substrate preparation,
+// simulation handles, state transfer and kernel dispatch.
+//
+// Built with -ffp-contract=off: the triangle records and grid membership are
+// computed with exactly the float operations the reference uses
+// (spinSystem.cpp:612-648, 1094-1379) so the device sees the same numbers.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/dsim.h"
+#include "dsim_internal.h"
+
+using dsim::kNone;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(DSIM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+struct V3 {
+    float x, y, z;
+};
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline float dotv(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        n = count;
+        if (count == 0) return DSIM_OK;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(DSIM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        return DSIM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+int upload(DevBuf<T> &d, const std::vector<T> &h) {
+    int rc = d.alloc(h.size());
+    if (rc) return rc;
+    if (!h.empty()) HIP_TRY(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return DSIM_OK;
+}
+
+inline float bits_to_float(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+unsigned n_workers() {
+    unsigned n = std::thread::hardware_concurrency();
+    if (n == 0) n = 1;
+    return std::min(n, 16u);
+}
+
+// ---------------------------------------------------------------------------
+// Triangle records
+// ---------------------------------------------------------------------------
+
+// m_triangleHelpers of spinSystem.cpp:612-648.
+void helpers_of(const float *V, const uint32_t *T, float *h) {
+    uint32_t p1 = T[0], p2 = T[1], p3 = T[2];
+    float x1 = V[p1 * 3 + 0], y1 = V[p1 * 3 + 1], z1 = V[p1 * 3 + 2];
+    float x2 = V[p2 * 3 + 0], y2 = V[p2 * 3 + 1], z2 = V[p2 * 3 + 2];
+    float x3 = V[p3 * 3 + 0], y3 = V[p3 * 3 + 1], z3 = V[p3 * 3 + 2];
+    float nx = (y2 - y1) * (z3 - z1) - (z2 - z1) * (y3 - y1);
+    float ny = (z2 - z1) * (x3 - x1) - (x2 - x1) * (z3 - z1);
+    float nz = (x2 - x1) * (y3 - y1) - (y2 - y1) * (x3 - x1);
+    float len = std::sqrt(nx * nx + ny * ny + nz * nz);
+    h[0] = nx / len; h[1] = ny / len; h[2] = nz / len;
+    h[3] = x3 - x1; h[4] = y3 - y1; h[5] = z3 - z1;
+    h[6] = x2 - x1; h[7] = y2 - y1; h[8] = z2 - z1;
+    h[9] = (x3 - x1) * (x2 - x1) + (y3 - y1) * (y2 - y1) + (z3 - z1) * (z2 - z1);
+    h[10] = (x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1) + (z3 - z1) * (z3 - z1);
+    h[11] = (x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1) + (z2 - z1) * (z2 - z1);
+}
+
+// ---------------------------------------------------------------------------
+// Reference grid membership (populateCubes, spinSystem.cpp:1094-1379)
+// ---------------------------------------------------------------------------
+
+bool point_in_box(V3 p, V3 mn, V3 mx) {
+    return ((p.x <= mx.x) && (p.x >= mn.x)) && ((p.y <= mx.y) && (p.y >= mn.y)) &&
+           ((p.z <= mx.z) && (p.z >= mn.z));
+}
+
+// rayTriangleIntersect, spinSystem.cpp:1360-1379.
+bool segment_hits_triangle(V3 r1, V3 r2, V3 t1, V3 t2, V3 t3) {
+    V3 u = t2 - t1, v = t3 - t1;
+    V3 n = v3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+    float nl = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    float s = 2.0f, t = 2.0f;
+    n.x = n.x / nl; n.y = n.y / nl; n.z = n.z / nl;
+    float r = dotv(n, t1 - r1) / dotv(n, r2 - r1);
+    if ((0 < r) & (r < 1)) {
+        V3 w = (r1 + (r2 - r1) * r) - t1;
+        float uv = dotv(u, v), wv = dotv(w, v), vv = dotv(v, v), wu = dotv(w, u), uu = dotv(u, u);
+        s = (uv * wv - vv * wu) / (uv * uv - uu * vv);
+        t = (uv * wu - uu * wv) / (uv * uv - uu * vv);
+    }
+    return (s >= 0) && (t >= 0) && (s + t <= 1);
+}
+
+// Corner c of a box: bit 0 picks max x, bit 1 max y, bit 2 max z.  The edge
+// and face tables follow the argument order of spinSystem.cpp:1189-1244.
+inline V3 corner(V3 mn, V3 mx, int c) {
+    return v3((c & 1) ? mx.x : mn.x, (c & 2) ? mx.y : mn.y, (c & 4) ? mx.z : mn.z);
+}
+const int kBoxEdges[12][2] = {{0, 1}, {0, 2}, {0, 4}, {6, 2}, {6, 4}, {6, 7},
+                              {3, 2}, {3, 1}, {3, 7}, {5, 7}, {5, 1}, {5, 4}};
+const int kBoxFaces[12][3] = {{0, 1, 2}, {0, 4, 2}, {0, 4, 1}, {6, 2, 4}, {6, 2, 7}, {6, 4, 7},
+                              {3, 2, 1}, {3, 7, 1}, {3, 7, 2}, {5, 4, 7}, {5, 1, 7}, {5, 4, 1}};
+
+bool triangle_in_box(V3 p1, V3 p2, V3 p3, V3 mn, V3 mx) {
+    if (point_in_box(p1, mn, mx) || point_in_box(p2, mn, mx) || point_in_box(p3, mn, mx))
+        return true;
+    for (const auto &e : kBoxEdges)
+        if (segment_hits_triangle(corner(mn, mx, e[0]), corner(mn, mx, e[1]), p1, p2, p3))
+            return true;
+    const V3 a[3] = {p1, p1, p2}, b[3] = {p2, p3, p3};
+    for (int s = 0; s < 3; ++s)
+        for (const auto &f : kBoxFaces)
+            if (segment_hits_triangle(a[s], b[s], corner(mn, mx, f[0]), corner(mn, mx, f[1]),
+                                      corner(mn, mx, f[2])))
+                return true;
+    return false;
+}
+
+// calcCubePos with the reference's division and std::min/std::max clamp.
+inline uint32_t ref_cell(float p, float lo, float cell, uint32_t n) {
+    float q = std::floor((p - lo) / cell);
+    float hi = (float)(n - 1);
+    float mn = (hi < q) ? hi : q;
+    float mx = (0.0f < mn) ? mn : 0.0f;
+    return (uint32_t)mx;
+}
+
+struct Pair {
+    uint32_t cell, tri;
+};
+
+// Runs fn(t, out) for every triangle over worker threads; the pair lists are
+// concatenated in triangle order.
+template <class Fn>
+std::vector<Pair> collect_pairs(uint32_t n_tri, Fn fn) {
+    unsigned nw = std::max(1u, std::min(n_workers(), (n_tri + 255) / 256));
+    std::vector<std::vector<Pair>> parts(nw);
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nw; ++w) {
+        th.emplace_back([&, w] {
+            uint32_t a = (uint32_t)((uint64_t)n_tri * w / nw);
+            uint32_t b = (uint32_t)((uint64_t)n_tri * (w + 1) / nw);
+            for (uint32_t t = a; t < b; ++t) fn(t, parts[w]);
+        });
+    }
+    for (auto &t : th) t.join();
+    size_t total = 0;
+    for (auto &p : parts) total += p.size();
+    std::vector<Pair> all;
+    all.reserve(total);
+    for (auto &p : parts) all.insert(all.end(), p.begin(), p.end());
+    return all;
+}
+
+// Stable counting sort by cell: lists keep ascending triangle order.
+void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<uint32_t> &start,
+                  std::vector<uint32_t> &tri, uint32_t &max_per_cell) {
+    start.assign((size_t)n_cells + 1, 0);
+    for (const Pair &p : pairs) start[p.cell + 1]++;
+    max_per_cell = 0;
+    for (uint32_t c = 0; c < n_cells; ++c) {
+        max_per_cell = std::max(max_per_cell, start[c + 1]);
+        start[c + 1] += start[c];
+    }
+    tri.assign(pairs.size(), 0);
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (const Pair &p : pairs) tri[fill[p.cell]++] = p.tri;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Substrate
+// ---------------------------------------------------------------------------
+
+struct dsim_substrate {
+    uint32_t n_tri = 0, n_vert = 0, n_fibers = 0;
+    std::vector<float> helpers;           // [n_tri*12]
+    std::vector<uint32_t> cell_start, cell_tri;
+    uint32_t n_cells = 0, max_per_cell = 0;
+    uint32_t kind = 0;
+    uint32_t dims[3] = {0, 0, 0};
+    float lo[3] = {0, 0, 0}, cell[3] = {0, 0, 0}, inv_cell[3] = {0, 0, 0};
+    DevBuf<float4> tri_ref, tri_mt, tri_aabb;
+    DevBuf<uint32_t> d_cell_start, d_cell_tri;
+    DevBuf<uint8_t> fiber_myelin;
+    int refs = 0;
+    bool on_device = false;
+    ~dsim_substrate() {
+        tri_ref.release(); tri_mt.release(); tri_aabb.release();
+        d_cell_start.release(); d_cell_tri.release(); fiber_myelin.release();
+    }
+};
+
+extern "C" const char *dsim_last_error(void) { return g_err.c_str(); }
+extern "C" int dsim_abi_version(void) { return DSIM_ABI_VERSION; }
+
+extern "C" int dsim_device_count(int *n) {
+    if (!n) return fail(DSIM_ERR_ARG, "n is NULL");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *n = (e == hipSuccess) ? c : 0;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_params *grid,
+                                     dsim_substrate **out) {
+    if (!mesh || !grid || !out) return fail(DSIM_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (mesh->n_triangles && (!mesh->vertices || !mesh->triangles || !mesh->tri_info))
+        return fail(DSIM_ERR_ARG, "mesh arrays missing");
+    const uint32_t nT = mesh->n_triangles, nV = mesh->n_vertices;
+    for (uint64_t k = 0; k < (uint64_t)nT * 3; ++k)
+        if (mesh->triangles[k] >= nV)
+            return fail(DSIM_ERR_ARG, "triangle %llu references vertex %u of %u",
+                        (unsigned long long)(k / 3), mesh->triangles[k], nV);
+    for (uint32_t t = 0; t < nT; ++t) {
+        if (mesh->tri_info[t * 3 + 1] > 2) return fail(DSIM_ERR_ARG, "triangle %u membrane > 2", t);
+        if (mesh->tri_info[t * 3] >= DSIM_NO_FIBER)
+            return fail(DSIM_ERR_ARG, "triangle %u fibre index too large", t);
+        if (mesh->tri_info[t * 3 + 2] >= nV) return fail(DSIM_ERR_ARG, "triangle %u P1 bad", t);
+    }
+    int rc = DSIM_OK;
+    int ndev = 0;
+    const bool have_device = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
+
+    auto *S = new dsim_substrate();
+    S->n_tri = nT;
+    S->n_vert = nV;
+    S->kind = grid->kind;
+    const float *V = mesh->vertices;
+    const uint32_t *T = mesh->triangles;
+    uint32_t nf = 0;
+    for (uint32_t t = 0; t < nT; ++t) nf = std::max(nf, mesh->tri_info[t * 3] + 1);
+    S->n_fibers = nf;
+
+    // Triangle helpers and the two 64-byte record layouts.
+    S->helpers.resize((size_t)nT * 12);
+    std::vector<float4> ref((size_t)nT * 4), mt((size_t)nT * 4), box((size_t)nT * 2);
+    for (uint32_t t = 0; t < nT; ++t) {
+        float *h = &S->helpers[(size_t)t * 12];
+        helpers_of(V, T + (size_t)t * 3, h);
+        uint32_t info = (mesh->tri_info[t * 3] & 0xFFFFu) | (mesh->tri_info[t * 3 + 1] << 16);
+        uint32_t P1 = mesh->tri_info[t * 3 + 2];
+        ref[t * 4 + 0] = make_float4(h[0], h[1], h[2], h[3]);
+        ref[t * 4 + 1] = make_float4(h[4], h[5], h[6], h[7]);
+        ref[t * 4 + 2] = make_float4(h[8], h[9], h[10], h[11]);
+        ref[t * 4 + 3] = make_float4(V[P1 * 3], V[P1 * 3 + 1], V[P1 * 3 + 2], bits_to_float(info));
+        uint32_t a = T[t * 3], b = T[t * 3 + 1], c = T[t * 3 + 2];
+        float v0x = V[a * 3], v0y = V[a * 3 + 1], v0z = V[a * 3 + 2];
+        float nd = std::fmaf(h[0], v0x, std::fmaf(h[1], v0y, h[2] * v0z));
+        mt[t * 4 + 0] = make_float4(h[0], h[1], h[2], nd);
+        mt[t * 4 + 1] = make_float4(v0x, v0y, v0z, bits_to_float(info));
+        mt[t * 4 + 2] = make_float4(V[b * 3] - v0x, V[b * 3 + 1] - v0y, V[b * 3 + 2] - v0z, 0.f);
+        mt[t * 4 + 3] = make_float4(V[c * 3] - v0x, V[c * 3 + 1] - v0y, V[c * 3 + 2] - v0z, 0.f);
+        float mn[3], mx[3];
+        for (int k = 0; k < 3; ++k) {
+            float p = V[a * 3 + k], q = V[b * 3 + k], r = V[c * 3 + k];
+            mn[k] = p; if (q < mn[k]) mn[k] = q; if (r < mn[k]) mn[k] = r;
+            mx[k] = p; if (q > mx[k]) mx[k] = q; if (r > mx[k]) mx[k] = r;
+        }
+        box[t * 2 + 0] = make_float4(mn[0], mn[1], mn[2], bits_to_float(info));
+        box[t * 2 + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
+    }
+    std::vector<uint8_t> myel(std::max(1u, nf), 0);
+    for (uint32_t t = 0; t < nT; ++t)
+        if (mesh->tri_info[t * 3 + 1] == 1) myel[mesh->tri_info[t * 3]] = 1;
+
+    // Uniform grid.
+    std::vector<Pair> pairs;
+    if (grid->kind == DSIM_GRID_REFERENCE) {
+        uint32_t n = grid->dims[0];
+        if (n == 0 || n > 1024) { delete S; return fail(DSIM_ERR_ARG, "reference grid n=%u", n); }
+        float L = 2.0f / (float)n;  // m_cubeLength, spinSystem.cpp:1115
+        for (int k = 0; k < 3; ++k) {
+            S->dims[k] = n; S->lo[k] = -1.0f; S->cell[k] = L; S->inv_cell[k] = 1.0f / L;
+        }
+        pairs = collect_pairs(nT, [&](uint32_t t, std::vector<Pair> &o) {
+            V3 p[3];
+            for (int j = 0; j < 3; ++j) {
+                uint32_t vi = T[t * 3 + j];
+                p[j] = v3(V[vi * 3], V[vi * 3 + 1], V[vi * 3 + 2]);
+            }
+            V3 mn = p[0], mx = p[0];
+            if (p[1].x < mn.x) mn.x = p[1].x; if (p[2].x < mn.x) mn.x = p[2].x;
+            if (p[1].y < mn.y) mn.y = p[1].y; if (p[2].y < mn.y) mn.y = p[2].y;
+            if (p[1].z < mn.z) mn.z = p[1].z; if (p[2].z < mn.z) mn.z = p[2].z;
+            if (p[1].x > mx.x) mx.x = p[1].x; if (p[2].x > mx.x) mx.x = p[2].x;
+            if (p[1].y > mx.y) mx.y = p[1].y; if (p[2].y > mx.y) mx.y = p[2].y;
+            if (p[1].z > mx.z) mx.z = p[1].z; if (p[2].z > mx.z) mx.z = p[2].z;
+            uint32_t c0[3] = {ref_cell(mn.x, -1.0f, L, n), ref_cell(mn.y, -1.0f, L, n),
+                              ref_cell(mn.z, -1.0f, L, n)};
+            uint32_t c1[3] = {ref_cell(mx.x, -1.0f, L, n), ref_cell(mx.y, -1.0f, L, n),
+                              ref_cell(mx.z, -1.0f, L, n)};
+            for (uint32_t xc = c0[0]; xc <= c1[0]; ++xc)
+                for (uint32_t yc = c0[1]; yc <= c1[1]; ++yc)
+                    for (uint32_t zc = c0[2]; zc <= c1[2]; ++zc) {
+                        V3 bmin = v3(-1 + (float)xc * L, -1 + (float)yc * L, -1 + (float)zc * L);
+                        V3 bmax = v3(-1 + (float)(xc + 1) * L, -1 + (float)(yc + 1) * L,
+                                     -1 + (float)(zc + 1) * L);
+                        if (triangle_in_box(p[0], p[1], p[2], bmin, bmax))
+                            o.push_back(Pair{(zc * n + yc) * n + xc, t});
+                    }
+        });
+    } else if (grid->kind == DSIM_GRID_AABB) {
+        for (int k = 0; k < 3; ++k) {
+            if (grid->dims[k] == 0 || !(grid->cell[k] > 0.0f)) {
+                delete S;
+                return fail(DSIM_ERR_ARG, "AABB grid dims/cell invalid on axis %d", k);
+            }
+            S->dims[k] = grid->dims[k]; S->lo[k] = grid->lo[k]; S->cell[k] = grid->cell[k];
+            S->inv_cell[k] = 1.0f / grid->cell[k];
+        }
+        if ((uint64_t)S->dims[0] * S->dims[1] * S->dims[2] > (1ull << 28)) {
+            delete S;
+            return fail(DSIM_ERR_ARG, "grid too large");
+        }
+        const float eps = grid->eps;
+        const uint32_t *dm = S->dims;
+        const float *lo = S->lo, *ce = S->cell;
+        pairs = collect_pairs(nT, [&](uint32_t t, std::vector<Pair> &o) {
+            float mn[3], mx[3];
+            int64_t i0[3], i1[3];
+            for (int k = 0; k < 3; ++k) {
+                float a = V[T[t * 3] * 3 + k], b = V[T[t * 3 + 1] * 3 + k], c = V[T[t * 3 + 2] * 3 + k];
+                mn[k] = a; if (b < mn[k]) mn[k] = b; if (c < mn[k]) mn[k] = c;
+                mx[k] = a; if (b > mx[k]) mx[k] = b; if (c > mx[k]) mx[k] = c;
+                mn[k] = mn[k] - eps; mx[k] = mx[k] + eps;
+                i0[k] = (int64_t)std::floor((mn[k] - lo[k]) / ce[k]) - 1;
+                i1[k] = (int64_t)std::floor((mx[k] - lo[k]) / ce[k]) + 1;
+                if (i0[k] < 0) i0[k] = 0;
+                if (i1[k] > (int64_t)dm[k] - 1) i1[k] = (int64_t)dm[k] - 1;
+                if (i0[k] > i1[k]) return;
+            }
+            for (int64_t z = i0[2]; z <= i1[2]; ++z) {
+                float bz0 = lo[2] + (float)z * ce[2], bz1 = lo[2] + (float)(z + 1) * ce[2];
+                if (bz0 > mx[2] || mn[2] > bz1) continue;
+                for (int64_t y = i0[1]; y <= i1[1]; ++y) {
+                    float by0 = lo[1] + (float)y * ce[1], by1 = lo[1] + (float)(y + 1) * ce[1];
+                    if (by0 > mx[1] || mn[1] > by1) continue;
+                    for (int64_t x = i0[0]; x <= i1[0]; ++x) {
+                        float bx0 = lo[0] + (float)x * ce[0], bx1 = lo[0] + (float)(x + 1) * ce[0];
+                        if (bx0 > mx[0] || mn[0] > bx1) continue;
+                        o.push_back(Pair{(uint32_t)((z * dm[1] + y) * dm[0] + x), t});
+                    }
+                }
+            }
+        });
+    } else {
+        delete S;
+        return fail(DSIM_ERR_ARG, "unknown grid kind %u", grid->kind);
+    }
+    S->n_cells = S->dims[0] * S->dims[1] * S->dims[2];
+    pairs_to_csr(pairs, S->n_cells, S->cell_start, S->cell_tri, S->max_per_cell);
+
+    if (!have_device) {  // host-only substrate: grid and helpers are inspectable, no sims
+        *out = S;
+        return DSIM_OK;
+    }
+    S->on_device = true;
+    if ((rc = upload(S->tri_ref, ref)) || (rc = upload(S->tri_mt, mt)) ||
+        (rc = upload(S->tri_aabb, box)) || (rc = upload(S->d_cell_start, S->cell_start)) ||
+        (rc = upload(S->fiber_myelin, myel))) {
+        delete S;
+        return rc;
+    }
+    std::vector<uint32_t> ct = S->cell_tri;
+    if (ct.empty()) ct.push_back(0);  // keep a valid pointer for empty grids
+    if ((rc = upload(S->d_cell_tri, ct))) {
+        delete S;
+        return rc;
+    }
+    *out = S;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_destroy(dsim_substrate *s) {
+    if (!s) return DSIM_OK;
+    if (s->refs > 0) return fail(DSIM_ERR_STATE, "substrate still used by %d sims", s->refs);
+    delete s;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cells,
+                                        uint32_t *n_entries, uint32_t *max_per_cell) {
+    if (!s) return fail(DSIM_ERR_ARG, "NULL substrate");
+    if (n_cells) *n_cells = s->n_cells;
+    if (n_entries) *n_entries = (uint32_t)s->cell_tri.size();
+    if (max_per_cell) *max_per_cell = s->max_per_cell;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_start,
+                                       uint32_t *cell_tri) {
+    if (!s) return fail(DSIM_ERR_ARG, "NULL substrate");
+    if (cell_start) memcpy(cell_start, s->cell_start.data(), s->cell_start.size() * 4);
+    if (cell_tri && !s->cell_tri.empty()) memcpy(cell_tri, s->cell_tri.data(), s->cell_tri.size() * 4);
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers) {
+    if (!s || !helpers) return fail(DSIM_ERR_ARG, "NULL argument");
+    if (!s->helpers.empty()) memcpy(helpers, s->helpers.data(), s->helpers.size() * 4);
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes) {
+    if (!s || !bytes) return fail(DSIM_ERR_ARG, "NULL argument");
+    *bytes = (uint64_t)s->tri_ref.n * 16 + s->tri_mt.n * 16 + s->tri_aabb.n * 16 +
+             s->d_cell_start.n * 4 + s->d_cell_tri.n * 4 + s->fiber_myelin.n;
+    return DSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Simulation handle
+// ---------------------------------------------------------------------------
+
+struct dsim_sim {
+    dsim_substrate *sub = nullptr;
+    dsim_params prm{};
+    uint32_t n = 0;
+    uint64_t step = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<float> x, y, z, mag, phase, disp, mom, momc, x0;
+    DevBuf<uint32_t> label, seed, flags, tries;
+    DevBuf<double> partial, out;
+    DevBuf<float> grads;
+    DevBuf<float4> inject;
+    DevBuf<uint32_t> trace;
+    float wsum = 0.0f;
+    bool moment_ready = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    double ms_done = 0.0;
+    uint64_t launches = 0;
+    ~dsim_sim() {
+        for (auto &e : ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        x.release(); y.release(); z.release(); mag.release(); phase.release(); disp.release();
+        mom.release(); momc.release(); x0.release(); label.release(); seed.release();
+        flags.release(); tries.release(); partial.release(); out.release(); grads.release();
+        inject.release(); trace.release();
+        if (stream) (void)hipStreamDestroy(stream);
+        if (sub) sub->refs--;
+    }
+};
+
+static int check_sim(dsim_sim *s) {
+    if (!s) return fail(DSIM_ERR_ARG, "NULL sim");
+    return DSIM_OK;
+}
+
+extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32_t n,
+                               dsim_sim **out) {
+    if (!sub || !p || !out) return fail(DSIM_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (!sub->on_device) return fail(DSIM_ERR_NODEV, "substrate was created without a HIP device");
+    if (p->mode > 1 || p->boundary > 1 || (p->rng != DSIM_RNG_MWC && p->rng != DSIM_RNG_PHILOX))
+        return fail(DSIM_ERR_ARG, "bad mode/boundary/rng");
+    if (p->mode == DSIM_MODE_REFERENCE && p->boundary != DSIM_BOUNDARY_CLAMP)
+        return fail(DSIM_ERR_ARG, "the reference mode clamps (spinKernelCpu.cpp:469-474)");
+    if (!(p->dt > 0.0f) || !(p->permeability >= 0.0f && p->permeability <= 1.0f))
+        return fail(DSIM_ERR_ARG, "dt must be > 0 and permeability in [0,1]");
+    auto *S = new dsim_sim();
+    S->sub = sub;
+    sub->refs++;
+    S->prm = *p;
+    S->n = n;
+    hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete S;
+        return fail(DSIM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    int rc;
+    size_t m = std::max<uint32_t>(n, 1);
+    if ((rc = S->x.alloc(m)) || (rc = S->y.alloc(m)) || (rc = S->z.alloc(m)) ||
+        (rc = S->mag.alloc(m)) || (rc = S->phase.alloc(m)) || (rc = S->label.alloc(m)) ||
+        (rc = S->flags.alloc(4)) || (rc = S->partial.alloc((size_t)dsim::kSignalBlocks * 15)) ||
+        (rc = S->out.alloc(15)) || (rc = S->tries.alloc(m))) {
+        delete S;
+        return rc;
+    }
+    if (p->rng == DSIM_RNG_MWC && (rc = S->seed.alloc(2 * m))) {
+        delete S;
+        return rc;
+    }
+    const std::pair<void *, size_t> zero[] = {
+        {S->flags.p, 16},    {S->mag.p, m * 4}, {S->phase.p, m * 4}, {S->x.p, m * 4},
+        {S->y.p, m * 4},     {S->z.p, m * 4},   {S->label.p, m * 4},
+        {S->seed.p, S->seed.p ? 2 * m * 4 : 0}};
+    for (const auto &zb : zero) {
+        if (!zb.first || !zb.second) continue;
+        e = hipMemsetAsync(zb.first, 0, zb.second, S->stream);
+        if (e != hipSuccess) {
+            delete S;
+            return fail(DSIM_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+        }
+    }
+    e = hipStreamSynchronize(S->stream);
+    if (e != hipSuccess) {
+        delete S;
+        return fail(DSIM_ERR_HIP, "init: %s", hipGetErrorString(e));
+    }
+    *out = S;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_sim_destroy(dsim_sim *s) {
+    if (!s) return DSIM_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    delete s;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_set_state_aos(dsim_sim *s, const float *pos, const dsim_spin_data *spins,
+                                  const uint32_t *seeds) {
+    if (int rc = check_sim(s)) return rc;
+    const uint32_t n = s->n;
+    if (n == 0) return DSIM_OK;
+    if (pos) {
+        std::vector<float> a(n), b(n), c(n);
+        for (uint32_t i = 0; i < n; ++i) { a[i] = pos[3 * i]; b[i] = pos[3 * i + 1]; c[i] = pos[3 * i + 2]; }
+        HIP_TRY(hipMemcpy(s->x.p, a.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->y.p, b.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->z.p, c.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    if (spins) {
+        std::vector<float> m(n), ph(n);
+        std::vector<uint32_t> lab(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (spins[i].compartmentType > 3)
+                return fail(DSIM_ERR_ARG, "spin %u compartment %u > 3", i, spins[i].compartmentType);
+            m[i] = spins[i].signalMagnitude;
+            ph[i] = spins[i].signalPhase;
+            lab[i] = (uint32_t)spins[i].compartmentType | ((uint32_t)spins[i].insideFiber << 8);
+        }
+        HIP_TRY(hipMemcpy(s->mag.p, m.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->phase.p, ph.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->label.p, lab.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    if (seeds) {
+        if (!s->seed.p) return fail(DSIM_ERR_STATE, "seeds given but rng is not MWC");
+        HIP_TRY(hipMemcpy(s->seed.p, seeds, (size_t)n * 8, hipMemcpyHostToDevice));
+    }
+    s->moment_ready = false;
+    return DSIM_OK;
+}
+
+static int sync_and_check(dsim_sim *s) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint32_t f[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpy(f, s->flags.p, 16, hipMemcpyDeviceToHost));
+    if (f[0] & 1u) {
+        HIP_TRY(hipMemset(s->flags.p, 0, 16));
+        return fail(DSIM_ERR_OVERFLOW, "a spin exceeded 4096 collisions in one step");
+    }
+    return DSIM_OK;
+}
+
+extern "C" int dsim_get_state_aos(dsim_sim *s, float *pos, dsim_spin_data *spins,
+                                  uint32_t *seeds) {
+    if (int rc = check_sim(s)) return rc;
+    if (int rc = sync_and_check(s)) return rc;
+    const uint32_t n = s->n;
+    if (n == 0) return DSIM_OK;
+    if (pos) {
+        std::vector<float> a(n), b(n), c(n);
+        HIP_TRY(hipMemcpy(a.data(), s->x.p, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b.data(), s->y.p, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c.data(), s->z.p, n * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) { pos[3 * i] = a[i]; pos[3 * i + 1] = b[i]; pos[3 * i + 2] = c[i]; }
+    }
+    if (spins) {
+        std::vector<float> m(n), ph(n);
+        std::vector<uint32_t> lab(n);
+        HIP_TRY(hipMemcpy(m.data(), s->mag.p, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ph.data(), s->phase.p, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(lab.data(), s->label.p, n * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            memset(&spins[i], 0, sizeof(dsim_spin_data));
+            spins[i].signalMagnitude = m[i];
+            spins[i].signalPhase = ph[i];
+            spins[i].compartmentType = (uint8_t)(lab[i] & 0xFFu);
+            spins[i].insideFiber = (uint16_t)(lab[i] >> 8);
+        }
+    }
+    if (seeds) {
+        if (!s->seed.p) return fail(DSIM_ERR_STATE, "rng is not MWC");
+        HIP_TRY(hipMemcpy(seeds, s->seed.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    }
+    return DSIM_OK;
+}
+
+extern "C" int dsim_set_step(dsim_sim *s, uint64_t step) {
+    if (int rc = check_sim(s)) return rc;
+    s->step = step;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_get_step(dsim_sim *s, uint64_t *step) {
+    if (int rc = check_sim(s)) return rc;
+    if (!step) return fail(DSIM_ERR_ARG, "NULL step");
+    *step = s->step;
+    return DSIM_OK;
+}
+
+static dsim::StepParams step_params(dsim_sim *s) {
+    dsim::StepParams P{};
+    dsim_substrate *b = s->sub;
+    P.tri_ref = b->tri_ref.p;
+    P.tri_mt = b->tri_mt.p;
+    P.cell_start = b->d_cell_start.p;
+    P.cell_tri = b->d_cell_tri.p;
+    P.fiber_myelin = b->fiber_myelin.p;
+    P.n_fibers = b->n_fibers;
+    for (int k = 0; k < 3; ++k) {
+        P.dims[k] = b->dims[k]; P.lo[k] = b->lo[k]; P.cell[k] = b->cell[k]; P.inv_cell[k] = b->inv_cell[k];
+    }
+    for (int c = 0; c < 4; ++c) {
+        P.std_dev[c] = s->prm.std_dev[c];
+        P.t2[c] = s->prm.t2[c];
+        P.decay[c] = s->prm.dt / s->prm.t2[c];
+    }
+    P.dt = s->prm.dt;
+    P.perm = s->prm.permeability;
+    P.reflection_type = s->prm.reflection_type;
+    P.clamp_hi = 1.0f - s->prm.spin_radius;
+    P.clamp_lo = -1.0f + s->prm.spin_radius;
+    P.n = s->n;
+    P.x = s->x.p; P.y = s->y.p; P.z = s->z.p; P.mag = s->mag.p; P.phase = s->phase.p;
+    P.label = s->label.p;
+    P.seed = s->seed.p;
+    P.disp = s->disp.p; P.mom = s->mom.p; P.momc = s->momc.p;
+    P.key0 = (uint32_t)s->prm.seed;
+    P.key1 = (uint32_t)(s->prm.seed >> 32);
+    P.flags = s->flags.p;
+    return P;
+}
+
+// Largest step count per launch: keeps single kernels short.
+static const uint32_t kMaxStepsPerLaunch = 256;
+
+static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, bool moment,
+                     const float *n3u, uint32_t *trace) {
+    const uint32_t n = s->n;
+    if (n3u) {
+        if ((uint64_t)iters * n * 16 > (1ull << 32))
+            return fail(DSIM_ERR_ARG, "injected sequence too large (%u x %u)", iters, n);
+        if (s->inject.n < (size_t)iters * n) {
+            s->inject.release();
+            if (int rc = s->inject.alloc((size_t)iters * n)) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(s->inject.p, n3u, (size_t)iters * n * 16, hipMemcpyHostToDevice,
+                               s->stream));
+    }
+    if (trace) {
+        if ((uint64_t)iters * n * 20 > (1ull << 32))
+            return fail(DSIM_ERR_ARG, "trace too large (%u x %u)", iters, n);
+        if (s->trace.n < (size_t)iters * n * 5) {
+            s->trace.release();
+            if (int rc = s->trace.alloc((size_t)iters * n * 5)) return rc;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint32_t done = 0;
+    while (done < iters) {
+        uint32_t k = std::min(kMaxStepsPerLaunch, iters - done);
+        P.step0 = s->step + done;
+        P.inject = n3u ? s->inject.p + (size_t)done * n : nullptr;
+        P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
+        hipError_t e = dsim::launch_step(P, k, (int)s->prm.mode,
+                                         s->prm.boundary == DSIM_BOUNDARY_PERIODIC, rng,
+                                         trace != nullptr, moment, s->stream);
+        if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
+        done += k;
+        s->launches++;
+    }
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    s->ev.emplace_back(e0, e1);
+    s->step += iters;
+    if (trace) {
+        HIP_TRY(hipMemcpyAsync(trace, s->trace.p, (size_t)iters * n * 20, hipMemcpyDeviceToHost,
+                               s->stream));
+        if (int rc = sync_and_check(s)) return rc;
+    }
+    return DSIM_OK;
+}
+
+static int rng_kind(dsim_sim *s, const float *n3u) {
+    if (n3u) return dsim::kRngInject;
+    return s->prm.rng == DSIM_RNG_MWC ? dsim::kRngMwc : dsim::kRngPhilox;
+}
+
+extern "C" int dsim_integrate(dsim_sim *s, const float g[3], float pc, uint32_t iters) {
+    return dsim_integrate_injected(s, g, pc, nullptr, iters, nullptr);
+}
+
+extern "C" int dsim_integrate_injected(dsim_sim *s, const float g[3], float pc, const float *n3u,
+                                       uint32_t iters, uint32_t *trace) {
+    if (int rc = check_sim(s)) return rc;
+    if (!g) return fail(DSIM_ERR_ARG, "NULL gradient");
+    dsim::StepParams P = step_params(s);
+    P.gx = g[0]; P.gy = g[1]; P.gz = g[2]; P.pc = pc;
+    return run_steps(s, P, iters, rng_kind(s, n3u), false, n3u, trace);
+}
+
+extern "C" int dsim_signal(dsim_sim *s, double out[5]) {
+    if (int rc = check_sim(s)) return rc;
+    if (!out) return fail(DSIM_ERR_ARG, "NULL out");
+    hipError_t e = dsim::launch_signal(s->n, s->mag.p, s->phase.p, s->label.p, s->partial.p,
+                                       dsim::kSignalBlocks, s->out.p, s->stream);
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "signal launch: %s", hipGetErrorString(e));
+    double v[15];
+    HIP_TRY(hipMemcpyAsync(v, s->out.p, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    if (int rc = sync_and_check(s)) return rc;
+    for (int k = 0; k < 5; ++k)
+        out[k] = v[10 + k] == 0.0 ? 0.0 : std::sqrt(v[k] * v[k] + v[5 + k] * v[5 + k]) / v[10 + k];
+    return DSIM_OK;
+}
+
+static int ensure_moment(dsim_sim *s) {
+    size_t m = 3 * (size_t)std::max<uint32_t>(s->n, 1);
+    int rc;
+    if (!s->disp.p && ((rc = s->disp.alloc(m)) || (rc = s->mom.alloc(m)) ||
+                       (rc = s->momc.alloc(m)) || (rc = s->x0.alloc(m))))
+        return rc;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_moment_reset(dsim_sim *s) {
+    if (int rc = check_sim(s)) return rc;
+    if (int rc = ensure_moment(s)) return rc;
+    hipError_t e = dsim::launch_moment_reset(s->n, s->x.p, s->y.p, s->z.p, s->x0.p, s->disp.p,
+                                             s->mom.p, s->momc.p, s->stream);
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "moment reset: %s", hipGetErrorString(e));
+    s->wsum = 0.0f;
+    s->moment_ready = true;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_integrate_moment(dsim_sim *s, float w, uint32_t iters) {
+    return dsim_integrate_moment_injected(s, w, nullptr, iters, nullptr);
+}
+
+extern "C" int dsim_integrate_moment_injected(dsim_sim *s, float w, const float *n3u,
+                                              uint32_t iters, uint32_t *trace) {
+    if (int rc = check_sim(s)) return rc;
+    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "call dsim_moment_reset first");
+    dsim::StepParams P = step_params(s);
+    P.weight = w;
+    int rc = run_steps(s, P, iters, rng_kind(s, n3u), true, n3u, trace);
+    if (rc == DSIM_OK) s->wsum += w * (float)iters;
+    return rc;
+}
+
+extern "C" int dsim_signal_moment(dsim_sim *s, uint32_t ndir, const float *g, float pc,
+                                  double *out) {
+    if (int rc = check_sim(s)) return rc;
+    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "no moments accumulated");
+    if (ndir == 0) return DSIM_OK;
+    if (!g || !out) return fail(DSIM_ERR_ARG, "NULL argument");
+    const uint32_t nb = 256;
+    if (s->grads.n < (size_t)ndir * 3) {
+        s->grads.release();
+        if (int rc = s->grads.alloc((size_t)ndir * 3)) return rc;
+    }
+    if (s->partial.n < (size_t)ndir * nb * 15) {
+        s->partial.release();
+        if (int rc = s->partial.alloc((size_t)std::max<size_t>(ndir * (size_t)nb, dsim::kSignalBlocks) * 15)) return rc;
+    }
+    DevBuf<double> o;
+    if (int rc = o.alloc((size_t)ndir * 15)) return rc;
+    HIP_TRY(hipMemcpyAsync(s->grads.p, g, (size_t)ndir * 12, hipMemcpyHostToDevice, s->stream));
+    hipError_t e = dsim::launch_moment_signal(s->n, s->mag.p, s->mom.p, s->x0.p, s->wsum,
+                                              s->label.p, ndir, s->grads.p, pc, s->partial.p, nb,
+                                              o.p, s->stream);
+    if (e != hipSuccess) { o.release(); return fail(DSIM_ERR_HIP, "moment signal: %s", hipGetErrorString(e)); }
+    std::vector<double> v((size_t)ndir * 15);
+    e = hipMemcpyAsync(v.data(), o.p, v.size() * 8, hipMemcpyDeviceToHost, s->stream);
+    int rc = sync_and_check(s);
+    o.release();
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "copy: %s", hipGetErrorString(e));
+    if (rc) return rc;
+    for (uint32_t j = 0; j < ndir; ++j) {
+        const double *q = &v[(size_t)j * 15];
+        for (int k = 0; k < 5; ++k)
+            out[j * 5 + k] = q[10 + k] == 0.0 ? 0.0 : std::sqrt(q[k] * q[k] + q[5 + k] * q[5 + k]) / q[10 + k];
+    }
+    return DSIM_OK;
+}
+
+extern "C" int dsim_get_moment(dsim_sim *s, float *moment, float *x0) {
+    if (int rc = check_sim(s)) return rc;
+    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "no moments accumulated");
+    if (int rc = sync_and_check(s)) return rc;
+    const uint32_t n = s->n;
+    std::vector<float> t(3 * (size_t)n);
+    if (moment) {
+        HIP_TRY(hipMemcpy(t.data(), s->mom.p, t.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) moment[3 * i + k] = t[(size_t)k * n + i];
+    }
+    if (x0) {
+        HIP_TRY(hipMemcpy(t.data(), s->x0.p, t.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) x0[3 * i + k] = t[(size_t)k * n + i];
+    }
+    return DSIM_OK;
+}
+
+extern "C" int dsim_init_spins_uniform(dsim_sim *s, float box) {
+    if (int rc = check_sim(s)) return rc;
+    hipError_t e = dsim::launch_init_uniform(s->n, s->x.p, s->y.p, s->z.p, s->mag.p, s->phase.p,
+                                             s->label.p, box, (uint32_t)s->prm.seed,
+                                             (uint32_t)(s->prm.seed >> 32), s->stream);
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
+    s->moment_ready = false;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
+                                uint32_t max_tries, uint32_t *n_ambiguous) {
+    if (int rc = check_sim(s)) return rc;
+    dsim::LabelParams L{};
+    L.tri_ref = s->sub->tri_ref.p;
+    L.tri_aabb = s->sub->tri_aabb.p;
+    L.n_tri = s->sub->n_tri;
+    L.n = s->n;
+    L.x = s->x.p; L.y = s->y.p; L.z = s->z.p;
+    L.label = s->label.p;
+    L.tries = s->tries.p;
+    L.xmax = xmax; L.ymax = ymax; L.zmax = zmax;
+    L.max_tries = max_tries;
+    L.plus_x = s->prm.mode == DSIM_MODE_PHYSICAL ? 1u : 0u;
+    L.flags = s->flags.p;
+    hipError_t e = dsim::launch_label(L, s->stream);
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "label launch: %s", hipGetErrorString(e));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint32_t f[4];
+    HIP_TRY(hipMemcpy(f, s->flags.p, 16, hipMemcpyDeviceToHost));
+    if (n_ambiguous) {
+        std::vector<uint32_t> t(s->n);
+        if (s->n) HIP_TRY(hipMemcpy(t.data(), s->tries.p, (size_t)s->n * 4, hipMemcpyDeviceToHost));
+        uint32_t a = 0;
+        for (uint32_t v : t) a += v > max_tries;
+        *n_ambiguous = a;
+    }
+    if (f[0] & 2u) {
+        HIP_TRY(hipMemset(s->flags.p, 0, 16));
+        return fail(DSIM_ERR_OVERFLOW, "two fibres of one membrane type enclose a spin");
+    }
+    s->moment_ready = false;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_sync(dsim_sim *s) {
+    if (int rc = check_sim(s)) return rc;
+    return sync_and_check(s);
+}
+
+extern "C" int dsim_step_timing(dsim_sim *s, double *ms, uint64_t *launches) {
+    if (int rc = check_sim(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (auto &e : s->ev) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e.first, e.second));
+        s->ms_done += t;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    s->ev.clear();
+    if (ms) *ms = s->ms_done;
+    if (launches) *launches = s->launches;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_step_timing_reset(dsim_sim *s) {
+    if (int rc = check_sim(s)) return rc;
+    double t;
+    if (int rc = dsim_step_timing(s, &t, nullptr)) return rc;
+    s->ms_done = 0.0;
+    s->launches = 0;
+    return DSIM_OK;
+}

@@ -1,0 +1,734 @@
+// HIP kernels of the dsim simulate path (gfx950).
+//
+// Built with -ffp-contract=off: every float expression rounds as written.
+// The reference-mode code keeps the evaluation order of dSim3's CPU path
+// (spinKernelCpu.cpp) so that, given identical normals, it reproduces that
+// path's hit triangles, cells, labels, positions and phases bit for bit; the
+// physical-mode code spells its fused multiply-adds out with fmaf().
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dsim_internal.h"
+
+namespace dsim {
+
+#define TWOPI_F 6.28318530717959f  // spinSystem.cpp:20
+
+// ------------------------------------------------------------------------ //
+// RNG
+// ------------------------------------------------------------------------ //
+
+// Marsaglia multiply-with-carry, spinKernelCpu.cpp:41-45
+__device__ __forceinline__ uint32_t mwc_next(uint32_t &s0, uint32_t &s1) {
+    s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
+    s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
+    return (s0 << 16) + s1;
+}
+
+// myRandn + boxMuller, spinKernelCpu.cpp:27-63 (ocml logf/cosf/sinf).
+__device__ __forceinline__ void mwc_randn(uint32_t &s0, uint32_t &s1, float &n1, float &n2,
+                                          float &n3, float &u) {
+    n1 = ((float)mwc_next(s0, s1) + 1.0f) / 4294967296.0f;
+    n2 = ((float)mwc_next(s0, s1) + 1.0f) / 4294967296.0f;
+    n3 = ((float)mwc_next(s0, s1) + 1.0f) / 4294967296.0f;
+    u = ((float)mwc_next(s0, s1) + 1.0f) / 4294967296.0f;
+    float r1 = sqrtf(-2.0f * logf(n1));
+    float p1 = TWOPI_F * n2;
+    float r2 = sqrtf(-2.0f * logf(n3));
+    float p2 = TWOPI_F * u;
+    n1 = r1 * cosf(p1);
+    n2 = r1 * sinf(p1);
+    n3 = r2 * cosf(p2);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__device__ __forceinline__ float u32_unit(uint32_t x) {
+    return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f;
+}
+
+// Gaussian normals from one Philox block with the hardware transcendental
+// units: v_log_f32 is log2, v_cos_f32/v_sin_f32 take revolutions, so
+// cos(2*pi*u) is one instruction for u in (0,1).
+__device__ __forceinline__ void philox_normals(uint32_t k0, uint32_t k1, uint32_t spin,
+                                               uint64_t step, float &n0, float &n1, float &n2) {
+    uint4 o = philox(make_uint4(spin, (uint32_t)step, (uint32_t)(step >> 32), 0u), k0, k1);
+    float u1 = u32_unit(o.x), u2 = u32_unit(o.y), u3 = u32_unit(o.z), u4 = u32_unit(o.w);
+    const float m2ln2 = -1.3862943611198906f;  // -2 ln 2
+    float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1));
+    float r2 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u3));
+    n0 = r1 * __builtin_amdgcn_cosf(u2);
+    n1 = r1 * __builtin_amdgcn_sinf(u2);
+    n2 = r2 * __builtin_amdgcn_cosf(u4);
+}
+
+__device__ __forceinline__ float philox_uniform(uint32_t k0, uint32_t k1, uint32_t spin,
+                                                uint64_t step) {
+    uint4 o = philox(make_uint4(spin, (uint32_t)step, (uint32_t)(step >> 32), 1u), k0, k1);
+    return u32_unit(o.x);
+}
+
+// ------------------------------------------------------------------------ //
+// Grid
+// ------------------------------------------------------------------------ //
+
+// calcCubePos, spinSystem.cpp:691-717: std::min/std::max clamp in float.
+__device__ __forceinline__ uint32_t clamp_cell(float q, uint32_t n) {
+    float hi = (float)(n - 1);
+    float mn = (hi < q) ? hi : q;
+    float mx = (0.0f < mn) ? mn : 0.0f;
+    return (uint32_t)mx;
+}
+
+template <int MODE>
+__device__ __forceinline__ void cell_of(const StepParams &P, float x, float y, float z,
+                                        uint32_t &cx, uint32_t &cy, uint32_t &cz) {
+    if (MODE == 0) {
+        cx = clamp_cell(floorf((x - P.lo[0]) / P.cell[0]), P.dims[0]);
+        cy = clamp_cell(floorf((y - P.lo[1]) / P.cell[1]), P.dims[1]);
+        cz = clamp_cell(floorf((z - P.lo[2]) / P.cell[2]), P.dims[2]);
+    } else {
+        cx = clamp_cell(floorf((x - P.lo[0]) * P.inv_cell[0]), P.dims[0]);
+        cy = clamp_cell(floorf((y - P.lo[1]) * P.inv_cell[1]), P.dims[1]);
+        cz = clamp_cell(floorf((z - P.lo[2]) * P.inv_cell[2]), P.dims[2]);
+    }
+}
+
+__device__ __forceinline__ uint32_t cell_hash(const StepParams &P, uint32_t cx, uint32_t cy,
+                                              uint32_t cz) {
+    return (cz * P.dims[1] + cy) * P.dims[0] + cx;
+}
+
+// ------------------------------------------------------------------------ //
+// Ray-triangle tests
+// ------------------------------------------------------------------------ //
+
+struct Hit {
+    uint32_t type;
+    float px, py, pz;
+    uint32_t idx;
+    float d;  // reference: squared distance; MT: parametric t
+};
+
+// triCollDetect, spinKernelCpu.cpp:354-395 (softsurfer test, reference order).
+__device__ __forceinline__ void test_ref(const float4 *__restrict__ R, uint32_t t, float ox,
+                                         float oy, float oz, float px, float py, float pz, Hit &h) {
+    h.type = 0;
+    const float4 a = R[t * 4 + 0];
+    const float4 d = R[t * 4 + 3];
+    float num = a.x * (d.x - ox) + a.y * (d.y - oy) + a.z * (d.z - oz);
+    float den = a.x * (px - ox) + a.y * (py - oy) + a.z * (pz - oz);
+    // r in (0,1) needs num and den of one sign with |num| <= |den|; the
+    // division (and the rest of the record) is only needed then.
+    if (!((num > 0.0f && den >= num) || (num < 0.0f && den <= num))) return;
+    float r = num / den;
+    if (!((0 < r) & (r < 1))) return;
+    const float4 b = R[t * 4 + 1];
+    const float4 c = R[t * 4 + 2];
+    float wu = (ox + r * (px - ox) - d.x) * a.w + (oy + r * (py - oy) - d.y) * b.x +
+               (oz + r * (pz - oz) - d.z) * b.y;
+    float wv = (ox + r * (px - ox) - d.x) * b.z + (oy + r * (py - oy) - d.y) * b.w +
+               (oz + r * (pz - oz) - d.z) * c.x;
+    float h9 = c.y, h10 = c.z, h11 = c.w;
+    float s = (h9 * wv - h11 * wu) / (h9 * h9 - h10 * h11);
+    float tt = (h9 * wu - h10 * wv) / (h9 * h9 - h10 * h11);
+    if (!((s >= 0) & (tt >= 0) & (s + tt <= 1))) return;
+    h.px = ox + (px - ox) * r;
+    h.py = oy + (py - oy) * r;
+    h.pz = oz + (pz - oz) * r;
+    h.d = (ox - h.px) * (ox - h.px) + (oy - h.py) * (oy - h.py) + (oz - h.pz) * (oz - h.pz);
+    h.idx = t;
+    h.type = 1;
+    if ((s == 0) | (tt == 0) | (s + tt == 1)) {
+        h.type = 2;
+        if (((s == 0) & (tt == 0)) | ((s == 0) & (tt == 1)) | ((s == 1) & (tt == 0))) h.type = 3;
+    }
+}
+
+// Plane-side prefilter, then Moller-Trumbore on the open segment 0<t<1.
+__device__ __forceinline__ void test_mt(const float4 *__restrict__ M, uint32_t t, float ox,
+                                        float oy, float oz, float px, float py, float pz, Hit &h) {
+    h.type = 0;
+    const float4 pl = M[t * 4 + 0];
+    float s0 = fmaf(pl.x, ox, fmaf(pl.y, oy, fmaf(pl.z, oz, -pl.w)));
+    float s1 = fmaf(pl.x, px, fmaf(pl.y, py, fmaf(pl.z, pz, -pl.w)));
+    if ((s0 > 0.0f) == (s1 > 0.0f)) return;
+    const float4 v0 = M[t * 4 + 1];
+    const float4 e1 = M[t * 4 + 2];
+    const float4 e2 = M[t * 4 + 3];
+    float dx = px - ox, dy = py - oy, dz = pz - oz;
+    float qx_ = fmaf(dy, e2.z, -(dz * e2.y));
+    float qy_ = fmaf(dz, e2.x, -(dx * e2.z));
+    float qz_ = fmaf(dx, e2.y, -(dy * e2.x));
+    float det = fmaf(e1.x, qx_, fmaf(e1.y, qy_, e1.z * qz_));
+    if (det == 0.0f) return;
+    float inv = 1.0f / det;
+    float tx = ox - v0.x, ty = oy - v0.y, tz = oz - v0.z;
+    float u = fmaf(tx, qx_, fmaf(ty, qy_, tz * qz_)) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return;
+    float cx = fmaf(ty, e1.z, -(tz * e1.y));
+    float cy = fmaf(tz, e1.x, -(tx * e1.z));
+    float cz = fmaf(tx, e1.y, -(ty * e1.x));
+    float v = fmaf(dx, cx, fmaf(dy, cy, dz * cz)) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return;
+    float tt = fmaf(e2.x, cx, fmaf(e2.y, cy, e2.z * cz)) * inv;
+    if (!(tt > 0.0f && tt < 1.0f)) return;
+    h.px = fmaf(tt, dx, ox);
+    h.py = fmaf(tt, dy, oy);
+    h.pz = fmaf(tt, dz, oz);
+    h.d = tt;
+    h.idx = t;
+    h.type = 1;
+    if (u == 0.0f || v == 0.0f || u + v == 1.0f) {
+        h.type = 2;
+        if ((u == 0.0f && v == 0.0f) || (u == 1.0f && v == 0.0f) || (u == 0.0f && v == 1.0f))
+            h.type = 3;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ bool better(const Hit &c, const Hit &b) {
+    if (c.type == 0) return false;
+    if (MODE == 0) return c.d < b.d;
+    if (b.type == 0) return true;
+    return (c.d < b.d) || (c.d == b.d && c.idx < b.idx);
+}
+
+// cubeCollDetect, spinKernelCpu.cpp:311-337, over one CSR cell.
+template <int MODE>
+__device__ __forceinline__ void cell_test(const StepParams &P, uint32_t cell, uint32_t excluded,
+                                          float ox, float oy, float oz, float px, float py,
+                                          float pz, Hit &best) {
+    uint32_t k0 = P.cell_start[cell], k1 = P.cell_start[cell + 1];
+    for (uint32_t k = k0; k < k1; ++k) {
+        uint32_t t = P.cell_tri[k];
+        if (t == excluded) continue;
+        Hit c;
+        if (MODE == 0)
+            test_ref(P.tri_ref, t, ox, oy, oz, px, py, pz, c);
+        else
+            test_mt(P.tri_mt, t, ox, oy, oz, px, py, pz, c);
+        if (better<MODE>(c, best)) best = c;
+    }
+}
+
+// reflectPos, spinKernelCpu.cpp:406-422.
+template <int MODE>
+__device__ __forceinline__ void reflect(const StepParams &P, float sx, float sy, float sz,
+                                        float tx, float ty, float tz, const Hit &h, float &rx,
+                                        float &ry, float &rz) {
+    if ((P.reflection_type == 0) | (h.type > 1)) {
+        rx = sx; ry = sy; rz = sz;
+        return;
+    }
+    if (MODE == 0) {
+        const float4 a = P.tri_ref[h.idx * 4];
+        float nx = a.x, ny = a.y, nz = a.z;
+        float qx = tx - h.px, qy = ty - h.py, qz = tz - h.pz;
+        rx = (1 - 2 * nx * nx) * qx - 2 * nx * ny * qy - 2 * nx * nz * qz + h.px;
+        ry = -2 * nx * ny * qx + (1 - 2 * ny * ny) * qy - 2 * ny * nz * qz + h.py;
+        rz = -2 * nx * nz * qx - 2 * ny * nz * qy + (1 - 2 * nz * nz) * qz + h.pz;
+    } else {
+        const float4 pl = P.tri_mt[h.idx * 4];
+        float qx = tx - h.px, qy = ty - h.py, qz = tz - h.pz;
+        float dn = fmaf(pl.x, qx, fmaf(pl.y, qy, pl.z * qz));
+        float k = -2.0f * dn;
+        rx = fmaf(k, pl.x, qx) + h.px;
+        ry = fmaf(k, pl.y, qy) + h.py;
+        rz = fmaf(k, pl.z, qz) + h.pz;
+    }
+}
+
+// Compartment change on transmission: reference table spinKernelCpu.cpp:251-270
+// (MODE 0); physical table (MODE 1) moves across each wall to the compartment
+// on its other side.
+template <int MODE>
+__device__ __forceinline__ void transit(const StepParams &P, uint32_t info, uint32_t &comp,
+                                        uint32_t &fiber) {
+    uint32_t f = info & 0xFFFFu, m = info >> 16;
+    if (MODE == 0) {
+        if (comp == 2) {
+            if (m == 0) comp = 1; else { comp = 0; fiber = kFiberNone; }
+        } else if (comp == 1) {
+            comp = 2;
+        } else if (comp == 3) {
+            comp = 0; fiber = kFiberNone;
+        } else {
+            fiber = f;
+            comp = (m == 1) ? 2 : 3;
+        }
+    } else {
+        uint32_t myel = (f < P.n_fibers) ? P.fiber_myelin[f] : 0u;
+        if (comp == 0) {
+            fiber = f;
+            comp = (m == 0) ? 1 : (m == 1) ? 2 : 3;
+        } else if (comp == 1) {
+            if (myel) comp = 2; else { comp = 0; fiber = kFiberNone; }
+        } else if (comp == 2) {
+            if (m == 0) comp = 1; else { comp = 0; fiber = kFiberNone; }
+        } else {
+            comp = 0; fiber = kFiberNone;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t tri_info_of(const StepParams &P, uint32_t t, int mode) {
+    return mode == 0 ? __float_as_uint(P.tri_ref[t * 4 + 3].w)
+                     : __float_as_uint(P.tri_mt[t * 4 + 1].w);
+}
+
+// collDetectRectGrid, spinKernelCpu.cpp:198-301.  MODE 0 stops at the first
+// cell (x-major walk) holding a hit; MODE 1 visits the whole cell range and
+// keeps the nearest hit (t, then triangle index).
+template <int MODE, int RNG>
+__device__ __forceinline__ void collide(const StepParams &P, float ox, float oy, float oz,
+                                        float &px, float &py, float &pz, float &u, bool &have_u,
+                                        uint32_t spin, uint64_t gstep, uint32_t &comp,
+                                        uint32_t &fiber, uint32_t &first, uint32_t &last,
+                                        uint32_t &ncoll) {
+    float ex = px, ey = py, ez = pz;
+    uint32_t excluded = kNone;
+    float umax = 1.0f, umin = 0.0f;
+    for (int guard = 0; guard < 4096; ++guard) {
+        uint32_t sx, sy, sz, tx, ty, tz;
+        cell_of<MODE>(P, ox, oy, oz, sx, sy, sz);
+        cell_of<MODE>(P, px, py, pz, tx, ty, tz);
+        uint32_t ix = (uint32_t)((tx > sx) - (tx < sx));
+        uint32_t iy = (uint32_t)((ty > sy) - (ty < sy));
+        uint32_t iz = (uint32_t)((tz > sz) - (tz < sz));
+        Hit best;
+        best.type = 0;
+        best.idx = kNone;
+        best.d = (MODE == 0) ? 400000000.0f : 2.0f;
+        best.px = best.py = best.pz = 0.0f;
+        uint32_t cx = sx;
+        do {
+            uint32_t cy = sy;
+            do {
+                uint32_t cz = sz;
+                do {
+                    uint32_t cell = cell_hash(P, cx, cy, cz);
+                    if (MODE == 0) {
+                        Hit c;
+                        c.type = 0;
+                        c.idx = kNone;
+                        c.d = 400000000.0f;
+                        cell_test<0>(P, cell, excluded, ox, oy, oz, px, py, pz, c);
+                        best = c;
+                    } else {
+                        cell_test<1>(P, cell, excluded, ox, oy, oz, px, py, pz, best);
+                    }
+                    cz += iz;
+                } while (cz != tz + iz && (MODE == 1 || best.type == 0));
+                cy += iy;
+            } while (cy != ty + iy && (MODE == 1 || best.type == 0));
+            cx += ix;
+        } while (cx != tx + ix && (MODE == 1 || best.type == 0));
+        if (best.type == 0) break;
+        if (ncoll == 0) first = best.idx;
+        last = best.idx;
+        ncoll += 1;
+        if (RNG == kRngPhilox && !have_u) {
+            u = philox_uniform(P.key0, P.key1, spin, gstep);
+            have_u = true;
+        }
+        float thr = umax - (umax - umin) * P.perm;
+        if (u <= thr) {
+            reflect<MODE>(P, ox, oy, oz, px, py, pz, best, ex, ey, ez);
+            umax = thr;
+        } else {
+            umin = thr;
+            transit<MODE>(P, tri_info_of(P, best.idx, MODE), comp, fiber);
+        }
+        ox = best.px; oy = best.py; oz = best.pz;
+        px = ex; py = ey; pz = ez;
+        excluded = best.idx;
+        if (guard == 4095) atomicOr(P.flags, 1u);
+    }
+    px = ex; py = ey; pz = ez;
+}
+
+// ------------------------------------------------------------------------ //
+// Step kernel: cpuIntegrate (spinKernelCpu.cpp:430-499) for `iters` steps
+// per launch with the spin's state held in registers.
+// ------------------------------------------------------------------------ //
+template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+__global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    float x = P.x[i], y = P.y[i], z = P.z[i];
+    float mag = P.mag[i];
+    float ph = MOMENT ? 0.0f : P.phase[i];
+    uint32_t lab = P.label[i];
+    uint32_t comp = lab & 0xFFu, fiber = (lab >> 8) & 0xFFFFu;
+    uint32_t s0 = 0, s1 = 0;
+    if (RNG == kRngMwc) { s0 = P.seed[2 * i]; s1 = P.seed[2 * i + 1]; }
+    float dx = 0.f, dy = 0.f, dz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    if (MOMENT) {
+        dx = P.disp[i]; dy = P.disp[P.n + i]; dz = P.disp[2 * P.n + i];
+        mx = P.mom[i]; my = P.mom[P.n + i]; mz = P.mom[2 * P.n + i];
+        cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
+    }
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint64_t gstep = P.step0 + it;
+        float n0, n1, n2, u = 0.0f;
+        bool have_u = true;
+        if (RNG == kRngInject) {
+            float4 q = P.inject[(size_t)it * P.n + i];
+            n0 = q.x; n1 = q.y; n2 = q.z; u = q.w;
+        } else if (RNG == kRngMwc) {
+            mwc_randn(s0, s1, n0, n1, n2, u);
+        } else {
+            philox_normals(P.key0, P.key1, i, gstep, n0, n1, n2);
+            have_u = false;
+        }
+        const float sd = comp == 0 ? P.std_dev[0] : comp == 1 ? P.std_dev[1]
+                       : comp == 2 ? P.std_dev[2] : P.std_dev[3];
+        const float ox = x, oy = y, oz = z;
+        float px, py, pz;
+        if (MODE == 0) {
+            px = x + n0 * sd; py = y + n1 * sd; pz = z + n2 * sd;
+        } else {
+            px = fmaf(n0, sd, x); py = fmaf(n1, sd, y); pz = fmaf(n2, sd, z);
+        }
+        uint32_t first = kNone, last = kNone, ncoll = 0;
+        collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, i, gstep, comp, fiber, first,
+                           last, ncoll);
+        if (!PERIODIC) {
+            // spinKernelCpu.cpp:469-474
+            if (px > P.clamp_hi) px = P.clamp_hi; else if (px < P.clamp_lo) px = P.clamp_lo;
+            if (py > P.clamp_hi) py = P.clamp_hi; else if (py < P.clamp_lo) py = P.clamp_lo;
+            if (pz > P.clamp_hi) pz = P.clamp_hi; else if (pz < P.clamp_lo) pz = P.clamp_lo;
+        }
+        if (MOMENT) {
+            dx += px - ox; dy += py - oy; dz += pz - oz;
+        }
+        if (PERIODIC) {
+            if (px >= 1.0f) px -= 2.0f; else if (px < -1.0f) px += 2.0f;
+            if (py >= 1.0f) py -= 2.0f; else if (py < -1.0f) py += 2.0f;
+            if (pz >= 1.0f) pz -= 2.0f; else if (pz < -1.0f) pz += 2.0f;
+        }
+        x = px; y = py; z = pz;
+        const float t2 = comp == 0 ? P.t2[0] : comp == 1 ? P.t2[1] : comp == 2 ? P.t2[2] : P.t2[3];
+        const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
+                       : comp == 2 ? P.decay[2] : P.decay[3];
+        if (MODE == 0) mag += -mag / t2 * P.dt;          // spinKernelCpu.cpp:486
+        else mag = fmaf(-mag, dc, mag);
+        if (!MOMENT) ph += (P.gx * x + P.gy * y + P.gz * z) * P.pc;  // spinKernelCpu.cpp:490
+        if (MOMENT && P.weight != 0.0f) {
+            float yv, tv;
+            yv = fmaf(P.weight, dx, -cx); tv = mx + yv; cx = (tv - mx) - yv; mx = tv;
+            yv = fmaf(P.weight, dy, -cy); tv = my + yv; cy = (tv - my) - yv; my = tv;
+            yv = fmaf(P.weight, dz, -cz); tv = mz + yv; cz = (tv - mz) - yv; mz = tv;
+        }
+        if (TRACE) {
+            uint32_t qx, qy, qz;
+            cell_of<MODE>(P, x, y, z, qx, qy, qz);
+            uint32_t *tr = P.trace + ((size_t)it * P.n + i) * 5;
+            tr[0] = first; tr[1] = last; tr[2] = ncoll;
+            tr[3] = cell_hash(P, qx, qy, qz);
+            tr[4] = comp | (fiber << 8);
+        }
+    }
+    P.x[i] = x; P.y[i] = y; P.z[i] = z;
+    P.mag[i] = mag;
+    if (!MOMENT) P.phase[i] = ph;
+    P.label[i] = comp | (fiber << 8);
+    if (RNG == kRngMwc) { P.seed[2 * i] = s0; P.seed[2 * i + 1] = s1; }
+    if (MOMENT) {
+        P.disp[i] = dx; P.disp[P.n + i] = dy; P.disp[2 * P.n + i] = dz;
+        P.mom[i] = mx; P.mom[P.n + i] = my; P.mom[2 * P.n + i] = mz;
+        P.momc[i] = cx; P.momc[P.n + i] = cy; P.momc[2 * P.n + i] = cz;
+    }
+}
+
+template <int MODE, bool PERIODIC, int RNG>
+static hipError_t launch3(const StepParams &p, uint32_t iters, bool trace, bool moment,
+                          hipStream_t s) {
+    dim3 grid((p.n + 255) / 256), block(256);
+    if (trace) {
+        if (moment) hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, true, true>), grid, block, 0, s, p, iters);
+        else hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, true, false>), grid, block, 0, s, p, iters);
+    } else {
+        if (moment) hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, false, true>), grid, block, 0, s, p, iters);
+        else hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, false, false>), grid, block, 0, s, p, iters);
+    }
+    return hipGetLastError();
+}
+
+template <int MODE, bool PERIODIC>
+static hipError_t launch2(const StepParams &p, uint32_t iters, int rng, bool trace, bool moment,
+                          hipStream_t s) {
+    switch (rng) {
+        case kRngInject: return launch3<MODE, PERIODIC, kRngInject>(p, iters, trace, moment, s);
+        case kRngMwc: return launch3<MODE, PERIODIC, kRngMwc>(p, iters, trace, moment, s);
+        default: return launch3<MODE, PERIODIC, kRngPhilox>(p, iters, trace, moment, s);
+    }
+}
+
+hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
+                       bool trace, bool moment, hipStream_t s) {
+    if (p.n == 0 || iters == 0) return hipSuccess;
+    if (mode == 0) return launch2<0, false>(p, iters, rng, trace, moment, s);
+    if (periodic) return launch2<1, true>(p, iters, rng, trace, moment, s);
+    return launch2<1, false>(p, iters, rng, trace, moment, s);
+}
+
+// ------------------------------------------------------------------------ //
+// Signal readout: getMrSignal(double[]) (spinSystem.cpp:1452-1500), FP64.
+// Per-block partials then one fixed-order final pass (deterministic).
+// ------------------------------------------------------------------------ //
+__device__ __forceinline__ void block_reduce15(double (&v)[15], double *out) {
+    __shared__ double sh[15][256];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 15; ++k) sh[k][t] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 15; ++k) sh[k][t] += sh[k][t + s];
+        __syncthreads();
+    }
+    if (t < 15) out[t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_signal_partial(uint32_t n, const float *mag,
+                                                        const float *phase, const uint32_t *label,
+                                                        double *partial) {
+    double v[15];
+    for (int k = 0; k < 15; ++k) v[k] = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double m = (double)mag[i];
+        double c = cos((double)phase[i]) * m, s = sin((double)phase[i]) * m;
+        uint32_t k = (label[i] & 0xFFu) + 1u;
+        v[0] += c; v[5] += s; v[10] += 1.0;
+        for (uint32_t q = 1; q < 5; ++q) {
+            bool on = (q == k);
+            v[q] += on ? c : 0.0;
+            v[5 + q] += on ? s : 0.0;
+            v[10 + q] += on ? 1.0 : 0.0;
+        }
+    }
+    block_reduce15(v, partial + (size_t)blockIdx.x * 15);
+}
+
+__global__ __launch_bounds__(256) void k_partial_final(const double *partial, uint32_t n_blocks,
+                                                       double *out) {
+    // out[group*15 + k] = sum over blocks of partial[(group*n_blocks + b)*15 + k]
+    const uint32_t g = blockIdx.x;
+    double v[15];
+    for (int k = 0; k < 15; ++k) v[k] = 0.0;
+    for (uint32_t b = threadIdx.x; b < n_blocks; b += 256)
+        for (int k = 0; k < 15; ++k) v[k] += partial[((size_t)g * n_blocks + b) * 15 + k];
+    block_reduce15(v, out + (size_t)g * 15);
+}
+
+hipError_t launch_signal(uint32_t n, const float *mag, const float *phase, const uint32_t *label,
+                         double *partial, uint32_t n_blocks, double *out15, hipStream_t s) {
+    hipLaunchKernelGGL(k_signal_partial, dim3(n_blocks), dim3(256), 0, s, n, mag, phase, label,
+                       partial);
+    hipLaunchKernelGGL(k_partial_final, dim3(1), dim3(256), 0, s, partial, n_blocks, out15);
+    return hipGetLastError();
+}
+
+// Per-direction readout from gradient moments: phase_j = pc * g_j.(M + W x0).
+__global__ __launch_bounds__(256) void k_moment_partial(uint32_t n, const float *mag,
+                                                        const float *mom, const float *x0,
+                                                        float wsum, const uint32_t *label,
+                                                        const float *grads, float pc,
+                                                        double *partial) {
+    const uint32_t j = blockIdx.y;
+    const float gx = grads[3 * j], gy = grads[3 * j + 1], gz = grads[3 * j + 2];
+    double v[15];
+    for (int k = 0; k < 15; ++k) v[k] = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float mx = mom[i], my = mom[n + i], mz = mom[2 * n + i];
+        if (x0 && wsum != 0.0f) {
+            mx = fmaf(wsum, x0[i], mx);
+            my = fmaf(wsum, x0[n + i], my);
+            mz = fmaf(wsum, x0[2 * n + i], mz);
+        }
+        float ph = fmaf(gx, mx, fmaf(gy, my, gz * mz)) * pc;
+        double m = (double)mag[i];
+        double c = cos((double)ph) * m, s = sin((double)ph) * m;
+        uint32_t k = (label[i] & 0xFFu) + 1u;
+        v[0] += c; v[5] += s; v[10] += 1.0;
+        for (uint32_t q = 1; q < 5; ++q) {
+            bool on = (q == k);
+            v[q] += on ? c : 0.0;
+            v[5 + q] += on ? s : 0.0;
+            v[10 + q] += on ? 1.0 : 0.0;
+        }
+    }
+    block_reduce15(v, partial + ((size_t)j * gridDim.x + blockIdx.x) * 15);
+}
+
+hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, const float *x0,
+                                float wsum, const uint32_t *label, uint32_t ndir,
+                                const float *grads, float pc, double *partial, uint32_t n_blocks,
+                                double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_moment_partial, dim3(n_blocks, ndir), dim3(256), 0, s, n, mag, mom, x0,
+                       wsum, label, grads, pc, partial);
+    hipLaunchKernelGGL(k_partial_final, dim3(ndir), dim3(256), 0, s, partial, n_blocks, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------ //
+// Labelling: findSpinsInFibers (spinSystem.cpp:1615-1852).  Every triangle
+// whose box overlaps the ray's box is a candidate (the R-tree query); the
+// crossing parity of each (fibre, membrane) group is folded into XOR sums so
+// the candidate order does not matter when at most one fibre per membrane
+// type is crossed an odd number of times (detected otherwise).
+// ------------------------------------------------------------------------ //
+__device__ __forceinline__ uint32_t fiber_hash(uint32_t f) { return f * 0x9E3779B1u + 0x7F4A7C15u; }
+
+__global__ __launch_bounds__(256) void k_label(LabelParams P) {
+    __shared__ float4 box[2][256];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < P.n;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
+    uint32_t tries = 0;
+    bool done = !live;
+    uint32_t out_comp = 0, out_fiber = kFiberNone;
+    while (__syncthreads_or(!done)) {
+        float ex = sx, ey = sy, ez = sz;
+        if (P.plus_x) {
+            ex = (float)(1.1 * (double)P.xmax);
+        } else if ((fabsf(sx) > fabsf(sy)) & (fabsf(sx) > fabsf(sz))) {
+            ex = (float)(((sx >= 0) - (sx < 0)) * 1.1 * (double)P.xmax);
+        } else if (fabsf(sy) > fabsf(sz)) {
+            ey = (float)(((sy >= 0) - (sy < 0)) * 1.1 * (double)P.ymax);
+        } else {
+            ez = (float)(((sz >= 0) - (sz < 0)) * 1.1 * (double)P.zmax);
+        }
+        if (tries > 0) {
+            uint4 o = philox(make_uint4(i, tries, 0u, 0u), 0x6A09E667u, 0xBB67AE85u);
+            ex = (float)((double)ex + (double)u32_unit(o.x) * 0.05);
+            ey = (float)((double)ey + (double)u32_unit(o.y) * 0.05);
+            ez = (float)((double)ez + (double)u32_unit(o.z) * 0.05);
+        }
+        float rminx = fminf(sx, ex), rminy = fminf(sy, ey), rminz = fminf(sz, ez);
+        float rmaxx = fmaxf(sx, ex), rmaxy = fmaxf(sy, ey), rmaxz = fmaxf(sz, ez);
+        uint32_t xf[3] = {0, 0, 0}, xh[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
+        for (uint32_t t0 = 0; t0 < P.n_tri; t0 += 256) {
+            __syncthreads();
+            uint32_t tl = t0 + threadIdx.x;
+            if (tl < P.n_tri) {
+                box[0][threadIdx.x] = P.tri_aabb[2 * tl];
+                box[1][threadIdx.x] = P.tri_aabb[2 * tl + 1];
+            }
+            __syncthreads();
+            if (done) continue;
+            uint32_t cnt = min(256u, P.n_tri - t0);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float4 lo = box[0][k], hi = box[1][k];
+                if (rminx > hi.x || lo.x > rmaxx || rminy > hi.y || lo.y > rmaxy ||
+                    rminz > hi.z || lo.z > rmaxz)
+                    continue;
+                Hit h;
+                test_ref(P.tri_ref, t0 + k, sx, sy, sz, ex, ey, ez, h);
+                if (h.type == 0) continue;
+                uint32_t info = __float_as_uint(lo.w);
+                uint32_t f = info & 0xFFFFu, m = info >> 16;
+                if (m > 2) continue;
+                xf[m] ^= f + 1u;
+                xh[m] ^= fiber_hash(f);
+                if (h.type >= 2) amb[m] += 1;
+            }
+        }
+        if (!done) {
+            uint32_t comp = 0, fib = kFiberNone, nedge = 0;
+            bool found = false;
+            for (uint32_t m = 0; m < 3 && !found; ++m) {
+                nedge += amb[m];
+                if (xf[m] != 0u) {
+                    uint32_t f = xf[m] - 1u;
+                    if (fiber_hash(f) != xh[m]) atomicOr(P.flags, 2u);
+                    found = true;
+                    fib = f;
+                    comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
+                } else if (xh[m] != 0u) {
+                    atomicOr(P.flags, 2u);
+                }
+            }
+            out_comp = comp;
+            out_fiber = fib;
+            if (nedge == 0) {
+                done = true;
+            } else {
+                tries++;
+                if (tries > P.max_tries) done = true;
+            }
+        }
+    }
+    if (live) {
+        P.label[i] = out_comp | (out_fiber << 8);
+        P.tries[i] = tries;
+    }
+}
+
+hipError_t launch_label(const LabelParams &p, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_label, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+// resetSpins position draw (spinSystem.cpp:1386-1398) from Philox stream 2.
+__global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, float *y, float *z,
+                                                      float *mag, float *phase, uint32_t *label,
+                                                      float box, uint32_t k0, uint32_t k1) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint4 o = philox(make_uint4(i, 0u, 0u, 2u), k0, k1);
+    float b2 = box * 2.0f;
+    x[i] = b2 * (u32_unit(o.x) - 0.5f);
+    y[i] = b2 * (u32_unit(o.y) - 0.5f);
+    z[i] = b2 * (u32_unit(o.z) - 0.5f);
+    mag[i] = 1.0f;
+    phase[i] = 0.0f;
+    label[i] = kFiberNone << 8;
+}
+
+hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
+                               uint32_t *label, float start_box, uint32_t key0, uint32_t key1,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_uniform, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, mag,
+                       phase, label, start_box, key0, key1);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_moment_reset(uint32_t n, const float *x, const float *y,
+                                                      const float *z, float *x0, float *disp,
+                                                      float *mom, float *momc) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    x0[i] = x[i]; x0[n + i] = y[i]; x0[2 * n + i] = z[i];
+    for (int k = 0; k < 3; ++k) {
+        disp[k * n + i] = 0.0f;
+        mom[k * n + i] = 0.0f;
+        momc[k * n + i] = 0.0f;
+    }
+}
+
+hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
+                               float *x0, float *disp, float *mom, float *momc, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_moment_reset, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, x0,
+                       disp, mom, momc);
+    return hipGetLastError();
+}
+
+}  // namespace dsim

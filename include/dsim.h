@@ -1,0 +1,201 @@
+/*
+ * dsim: MI355X-native diffusion-MRI Monte Carlo simulate path (C ABI).
+ *
+ * This header is the drop-in boundary that replaces the reference's
+ * `extern "C"` CUDA host glue (dSim3/src/spinSystem.cuh:1-59, defined in
+ * dSim3/src/spinSystem.cu:18-299) and the SpinSystem methods that feed it.
+ * Every entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Plain C types only: pointers, sizes, PODs.  No device pointers cross
+ *     the boundary unless a function says so.
+ *   - Every function returns a status: DSIM_OK (0) or a negative DSIM_ERR_*;
+ *     dsim_last_error() returns a message for the calling thread.  Nothing
+ *     calls exit() (the reference's CUDA_SAFE_CALL does, spinSystem.cu:38-98).
+ *   - Units at the boundary are the reference's: coordinates normalised to
+ *     [-1,1] (micrometres / spaceScale, spinSystem.cpp:422-424), gradients as
+ *     gradScaled = G[mT/m] * 1e-9 * spaceScale (spinSystem.cpp:1931) and
+ *     phaseConstant = 2*pi*gamma*dt (spinSystem.cpp:1938).
+ *   - A dsim_sim owns its device buffers and one HIP stream; calls on
+ *     different handles may run from different host threads.  Calls on one
+ *     handle must not overlap.
+ */
+#ifndef DSIM_H
+#define DSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSIM_ABI_VERSION 1
+
+#define DSIM_OK 0
+#define DSIM_ERR_ARG (-1)      /* bad argument or size */
+#define DSIM_ERR_HIP (-2)      /* HIP runtime error (message has the HIP string) */
+#define DSIM_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define DSIM_ERR_STATE (-4)    /* call not valid in the handle's current mode */
+#define DSIM_ERR_OVERFLOW (-5) /* a device-side bound was exceeded (flagged, not faulted) */
+#define DSIM_ERR_IO (-6)       /* file could not be opened */
+#define DSIM_ERR_FORMAT (-7)   /* input file rejected (same rules as the reference) */
+#define DSIM_ERR_NODEV (-8)    /* no HIP device visible */
+
+#define DSIM_NO_TRIANGLE 0xFFFFFFFFu
+#define DSIM_NO_FIBER 65535u
+
+/* spinData, dSim3/src/dSimDataTypes.h:9-24 (12 bytes with padding). */
+typedef struct {
+    float signalMagnitude;
+    float signalPhase;
+    uint8_t compartmentType; /* 0 extra, 1 intra-axonal, 2 myelin, 3 glia interior */
+    uint16_t insideFiber;    /* DSIM_NO_FIBER when outside every fibre */
+} dsim_spin_data;
+
+/* Mesh arrays as SpinSystem::initFibers leaves them
+ * (m_vertices, m_trgls, m_hTriInfo; spinSystem.cpp:290-560). */
+typedef struct {
+    uint32_t n_vertices;
+    const float *vertices;    /* [n_vertices*3], normalised */
+    uint32_t n_triangles;
+    const uint32_t *triangles; /* [n_triangles*3], 0-based vertex indices */
+    const uint32_t *tri_info;  /* [n_triangles*3]: fibre, membrane (0 A, 1 M, 2 G), first vertex */
+} dsim_mesh;
+
+enum {
+    DSIM_GRID_REFERENCE = 0, /* n^3 cubes over [-1,1]^3, reference membership (populateCubes) */
+    DSIM_GRID_AABB = 1       /* any box, triangle-AABB membership (conservative) */
+};
+
+typedef struct {
+    uint32_t kind;
+    uint32_t dims[3]; /* REFERENCE: dims[0]=n (the reference hard-codes 50); AABB: cells per axis */
+    float lo[3];      /* AABB: grid origin (normalised units) */
+    float cell[3];    /* AABB: cell edge per axis */
+    float eps;        /* AABB: membership margin */
+} dsim_grid_params;
+
+enum {
+    DSIM_MODE_REFERENCE = 0, /* the reference's CPU semantics (spinKernelCpu.cpp) */
+    DSIM_MODE_PHYSICAL = 1   /* Moller-Trumbore, nearest hit, physical compartment table */
+};
+enum { DSIM_BOUNDARY_CLAMP = 0, DSIM_BOUNDARY_PERIODIC = 1 };
+enum { DSIM_RNG_MWC = 1, DSIM_RNG_PHILOX = 2 };
+
+typedef struct {
+    float dt;                 /* ms (sim.timeStep) */
+    float std_dev[4];         /* per compartment, normalised (spinSystem.cpp:52-56) */
+    float t2[4];              /* ms per compartment, INFINITY = no decay (spinSystem.cpp:47-51) */
+    float permeability;       /* transmission probability per hit (sim.permeability) */
+    uint32_t reflection_type; /* 1 = specular (diffusion.cpp:96), 0 = back to start */
+    uint32_t mode;            /* DSIM_MODE_* */
+    uint32_t boundary;        /* DSIM_BOUNDARY_*; the reference mode always clamps */
+    uint32_t rng;             /* DSIM_RNG_*; MWC is the reference's per-spin stream */
+    uint64_t seed;            /* Philox key */
+    float spin_radius;        /* clamp margin (spinSystem.cpp:61: 0.005) */
+} dsim_params;
+
+typedef struct dsim_substrate dsim_substrate;
+typedef struct dsim_sim dsim_sim;
+
+/* ---- library ------------------------------------------------------------ */
+const char *dsim_last_error(void);
+int dsim_abi_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int dsim_device_count(int *n);
+
+/* ---- substrate ---------------------------------------------------------- */
+/* Replaces the device half of SpinSystem::initFibers (triangle helpers and
+ * their texture binds, spinSystem.cpp:612-665) and SpinSystem::populateCubes
+ * with its cube-array copies (spinSystem.cpp:1094-1352; bindVertices,
+ * bindTriangleHelpers, bindTriInfo, bindCubeCounter, bindTrianglesInCubes in
+ * spinSystem.cu:127-215).  The grid is stored as CSR (cell_start, cell_tri)
+ * in place of the reference's dense padded table; lists keep the reference's
+ * ascending triangle order. */
+int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_params *grid,
+                          dsim_substrate **out);
+int dsim_substrate_destroy(dsim_substrate *s);
+/* Grid sizes: cells, list entries, and the reference's m_maxTrianglesPerCube. */
+int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cells, uint32_t *n_entries,
+                             uint32_t *max_per_cell);
+/* Copies the CSR grid to host arrays (cell_start[n_cells+1], cell_tri[n_entries]). */
+int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_start, uint32_t *cell_tri);
+/* Copies m_triangleHelpers (12 floats per triangle, spinSystem.cpp:612-648). */
+int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers);
+/* Device bytes held by the substrate. */
+int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes);
+
+/* ---- simulation handle --------------------------------------------------- */
+/* Replaces SpinSystem::build's device allocations and constant uploads
+ * (spinSystem.cpp:178-281: allocateArray, copyConstantToDevice). */
+int dsim_sim_create(dsim_substrate *s, const dsim_params *p, uint32_t n_spins, dsim_sim **out);
+int dsim_sim_destroy(dsim_sim *sim);
+/* setArray + setSpinArray (spinSystem.cpp:1584-1606): host AoS positions
+ * [n*3], spinData [n], and MWC seeds [n*2] (may be NULL unless rng==MWC). */
+int dsim_set_state_aos(dsim_sim *sim, const float *pos, const dsim_spin_data *spins,
+                       const uint32_t *seeds);
+/* getArray + getSpinArray (spinSystem.cpp:1544-1571); any pointer may be NULL. */
+int dsim_get_state_aos(dsim_sim *sim, float *pos, dsim_spin_data *spins, uint32_t *seeds);
+/* Sets the Philox step counter (the next integrated step uses this value). */
+int dsim_set_step(dsim_sim *sim, uint64_t step);
+int dsim_get_step(dsim_sim *sim, uint64_t *step);
+
+/* resetSpins' position draw (spinSystem.cpp:1386-1398), on the device:
+ * positions uniform in [-start_box, start_box]^3 from the Philox stream,
+ * magnitude 1, phase 0, labels cleared.  Follow with dsim_label_spins. */
+int dsim_init_spins_uniform(dsim_sim *sim, float start_box);
+
+/* findSpinsInFibers (spinSystem.cpp:1615-1852), on the device.  The reference
+ * mode casts its ray along the spin's dominant axis to 1.1*max (as the
+ * reference does); the physical mode casts it along +x so that open prisms
+ * are counted.  Ambiguous rays (edge or vertex hits) are re-cast with a
+ * reproducible jitter up to max_tries times; spins still ambiguous keep their
+ * last label and are counted in *n_ambiguous.  Returns DSIM_ERR_OVERFLOW if a
+ * ray found an odd crossing count on two fibres of one membrane type. */
+int dsim_label_spins(dsim_sim *sim, float xmax, float ymax, float zmax, uint32_t max_tries,
+                     uint32_t *n_ambiguous);
+
+/* integrateSystem (spinSystem.cuh:34-45, spinSystem.cu:221-270): advance all
+ * spins `iterations` steps under a constant gradient, phase += g.pos *
+ * phaseConstant per step, state left on the device. */
+int dsim_integrate(dsim_sim *sim, const float grad_scaled[3], float phase_constant,
+                   uint32_t iterations);
+/* Test entry: as dsim_integrate but the normals and uniform of every step come
+ * from n3u (host [iterations][n][4]) and, if trace is not NULL, the per-step
+ * outcome is written to trace (host [iterations][n][5]: first hit triangle,
+ * last hit triangle, hits, cell of the end position, compartment|fibre<<8). */
+int dsim_integrate_injected(dsim_sim *sim, const float grad_scaled[3], float phase_constant,
+                            const float *n3u, uint32_t iterations, uint32_t *trace);
+
+/* getMrSignal(double[]) (spinSystem.cpp:1452-1500): out[0] total, out[1+c]
+ * compartment c, |sum m e^{i phase}| / count, FP64 sum. */
+int dsim_signal(dsim_sim *sim, double out[5]);
+
+/* ---- shared-walk gradient moments (one walk serves every direction) ------ */
+/* Zeroes the displacement and moment accumulators and records x0. */
+int dsim_moment_reset(dsim_sim *sim);
+/* Advance `iterations` steps accumulating M += weight * (x - x0) per step,
+ * weight in {+1, 0, -1} for the PGSE lobes of diffusion.cpp:564-585. */
+int dsim_integrate_moment(dsim_sim *sim, float weight, uint32_t iterations);
+/* Same, with injected normals/uniforms and optional trace (as above). */
+int dsim_integrate_moment_injected(dsim_sim *sim, float weight, const float *n3u,
+                                   uint32_t iterations, uint32_t *trace);
+/* Signal per direction j: phase = phase_constant * g_j . (M + W x0), FP64
+ * sum; out[j*5 + k] as in dsim_signal.  grads_scaled is [ndir*3]. */
+int dsim_signal_moment(dsim_sim *sim, uint32_t ndir, const float *grads_scaled,
+                       float phase_constant, double *out);
+/* Host copy of the moment accumulators [n*3] and x0 [n*3] (either may be NULL). */
+int dsim_get_moment(dsim_sim *sim, float *moment, float *x0);
+
+/* ---- timing ------------------------------------------------------------- */
+int dsim_sync(dsim_sim *sim);
+/* Milliseconds spent in step-kernel launches since the last reset, measured
+ * with HIP events on the handle's stream; launches counts kernel launches. */
+int dsim_step_timing(dsim_sim *sim, double *ms, uint64_t *launches);
+int dsim_step_timing_reset(dsim_sim *sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
