@@ -564,6 +564,7 @@ extern "C" int dsim_set_state_aos(dsim_sim *s, const float *pos, const dsim_spin
     if (int rc = check_sim(s)) return rc;
     const uint32_t n = s->n;
     if (n == 0) return DSIM_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));  // the copies below use the null stream
     if (pos) {
         std::vector<float> a(n), b(n), c(n);
         for (uint32_t i = 0; i < n; ++i) { a[i] = pos[3 * i]; b[i] = pos[3 * i + 1]; c[i] = pos[3 * i + 2]; }

@@ -123,7 +123,7 @@ def _declare(L):
     L.orc_coll_detect_brute.restype = F3
     L.orc_integrate.argtypes = [P(WorldC), P(StateC), C.c_uint32, P(StepArgsC)]
     L.orc_find_spins.argtypes = [P(WorldC), _fp, C.c_uint32, C.c_float, C.c_float, C.c_float,
-                                 C.c_uint32, _u8p, _u16p, _up]
+                                 C.c_uint32, C.c_uint32, _u8p, _u16p, _up]
     L.orc_find_spins.restype = C.c_uint32
     L.orc_mr_signal.argtypes = [C.c_uint32, _fp, _fp, _u8p, P(C.c_double)]
     L.orc_moment_signal.argtypes = [C.c_uint32, _fp, _fp, _fp, C.c_float, _u8p, C.c_uint32, _fp,
@@ -364,14 +364,14 @@ class World:
         return (np.array([r.x, r.y, r.z], np.float32), c8.value, f16.value,
                 (h.first, h.last, h.ncoll))
 
-    def find_spins(self, pos, xmax=1.0, ymax=1.0, zmax=1.0, max_tries=20):
+    def find_spins(self, pos, xmax=1.0, ymax=1.0, zmax=1.0, max_tries=20, plus_x=False):
         P = np.ascontiguousarray(pos, np.float32).ravel()
         n = P.size // 3
         comp = np.zeros(n, np.uint8)
         fib = np.zeros(n, np.uint16)
         tries = np.zeros(n, np.uint32)
         amb = lib().orc_find_spins(C.byref(self.c), _p(P, C.c_float), n, xmax, ymax, zmax,
-                                   max_tries, _p(comp, C.c_uint8), _p(fib, C.c_uint16),
+                                   max_tries, int(plus_x), _p(comp, C.c_uint8), _p(fib, C.c_uint16),
                                    _p(tries, C.c_uint32))
         return comp, fib, tries, amb
 

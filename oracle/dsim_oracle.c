@@ -720,8 +720,8 @@ static const uint32_t k_label_key[2] = {0x6A09E667u, 0xBB67AE85u};
  * last attempt's label (the reference instead moves it to a new random
  * position, spinSystem.cpp:1628-1633). */
 uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float xmax, float ymax,
-                        float zmax, uint32_t max_tries, uint8_t *comp_out, uint16_t *fiber_out,
-                        uint32_t *tries_out)
+                        float zmax, uint32_t max_tries, uint32_t plus_x, uint8_t *comp_out,
+                        uint16_t *fiber_out, uint32_t *tries_out)
 {
     uint32_t nT = w->n_tri;
     const float *V = w->vertices;
@@ -749,7 +749,9 @@ uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float 
         uint16_t fib = ORC_FIBER_NONE;
         for (;;) {
             orc_f3 le = sp;
-            if ((fabsf(sp.x) > fabsf(sp.y)) & (fabsf(sp.x) > fabsf(sp.z)))
+            if (plus_x)  /* build-defined: ray along +x (open prisms parallel to z) */
+                le.x = (float)(1.1 * (double)xmax);
+            else if ((fabsf(sp.x) > fabsf(sp.y)) & (fabsf(sp.x) > fabsf(sp.z)))
                 le.x = (float)(((sp.x >= 0) - (sp.x < 0)) * 1.1 * (double)xmax);
             else if (fabsf(sp.y) > fabsf(sp.z))
                 le.y = (float)(((sp.y >= 0) - (sp.y < 0)) * 1.1 * (double)ymax);

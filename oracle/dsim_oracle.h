@@ -168,7 +168,7 @@ void orc_integrate(const orc_world *w, orc_state *s, uint32_t iters,
  * spins whose labels stayed ambiguous after max_tries. */
 uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n,
                         float xmax, float ymax, float zmax, uint32_t max_tries,
-                        uint8_t *comp, uint16_t *fiber, uint32_t *tries_out);
+                        uint32_t plus_x, uint8_t *comp, uint16_t *fiber, uint32_t *tries_out);
 void orc_mr_signal(uint32_t n, const float *mag, const float *phase,
                    const uint8_t *comp, double out[5]);
 /* Build-defined: per-direction signal from gradient moments. */

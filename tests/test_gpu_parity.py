@@ -1,0 +1,244 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on identical
+inputs.  Injected step sequences must match bit for bit (hit triangles, cells,
+labels, positions, phases, magnitudes); free-running streams must agree within
+the stated tolerances."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dsim_amd import lib as D
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 42576.0
+DT = 0.005
+PC = np.float32(2 * np.pi * GAMMA * DT)  # phaseConstant, spinSystem.cpp:1938
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if D.device_count() == 0:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+
+
+def _mesh(ref_inputs, name, scale):
+    return O.parse_mesh(os.path.join(ref_inputs, name), scale)
+
+
+def _pair(m, *, mode=0, grid="reference", dims=None, lo=None, cell=None, eps=1e-5,
+          std=(0.03, 0.03, 0.01, 0.03), t2=(80.0, 70.0, 7.5, 70.0), perm=0.2, boundary=0,
+          rng=D.RNG_PHILOX, n=0, seed=1):
+    kw = dict(std_dev=std, t2=t2, dt=DT, permeability=perm)
+    if grid == "reference":
+        w = O.World(m["vertices"], m["trgls"], m["tri_info"], **kw)
+        sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"])
+    else:
+        w = O.World(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims, lo=lo,
+                    cell=cell, eps=eps, **kw)
+        sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims,
+                          lo=lo, cell=cell, eps=eps)
+    if mode == 1:
+        w.set(test=1, traversal=1, transitions=1, boundary=boundary)
+    sim = D.Simulation(sub, n, dt=DT, std_dev=std, t2=t2, permeability=perm, mode=mode,
+                       boundary=boundary, rng=rng, seed=seed)
+    return w, sub, sim
+
+
+def _spins(w, m, n, seed, box=0.9, plus_x=False):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-box, box, (n, 3)).astype(np.float32)
+    comp, fib, _, _ = w.find_spins(pos, m["xmax"], m["ymax"], m["zmax"], plus_x=plus_x)
+    return pos, comp, fib
+
+
+def _inject(n, iters, seed):
+    rng = np.random.default_rng(seed)
+    q = np.empty((iters, n, 4), np.float32)
+    q[..., :3] = rng.standard_normal((iters, n, 3)).astype(np.float32)
+    q[..., 3] = (rng.integers(0, 2 ** 32, (iters, n), dtype=np.uint64).astype(np.float32) + 1.0) / 4294967296.0
+    return q
+
+
+CASES = [("sim.fibers_1_r004_f09", 6.0), ("sim.fibers_1_r0064_f05", 8.0), ("sim.triangles_curved", 1.0)]
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_injected_reference_mode_bit_exact(ref_inputs, name, scale):
+    _say(f"case reference-injected {name} scale={scale}")
+    m = _mesh(ref_inputs, name, scale)
+    n, iters = 3000, 40
+    w, sub, sim = _pair(m, n=n)
+    pos, comp, fib = _spins(w, m, n, 7)
+    grad = np.array([0.05, -0.02, 0.01], np.float32)
+    q = _inject(n, iters, 8)
+    st = O.State(pos, comp=comp, fiber=fib)
+    tr_o = O.integrate(w, st, iters, rng=O.RNG_INJECT, inject=q, grad=grad, phase_constant=PC,
+                       trace=True)
+    sim.set_state(pos=pos, comp=comp, fiber=fib)
+    tr_g = sim.integrate_injected(grad, PC, q, trace=True)
+    g = sim.get_state()
+    assert (tr_o[..., 2] > 0).sum() > 500, "too few collisions to be a meaningful check"
+    np.testing.assert_array_equal(tr_g, tr_o)
+    np.testing.assert_array_equal(g["pos"], st.pos)
+    np.testing.assert_array_equal(g["phase"], st.phase)
+    np.testing.assert_array_equal(g["mag"], st.mag)
+    np.testing.assert_array_equal(g["comp"], st.comp)
+    np.testing.assert_array_equal(g["fiber"], st.fiber)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("name,scale", CASES[:2] + [("sim.triangles_curved", 1.0)])
+def test_injected_physical_mode_bit_exact(ref_inputs, name, scale, boundary):
+    _say(f"case physical-injected {name} scale={scale} boundary={boundary}")
+    m = _mesh(ref_inputs, name, scale)
+    n, iters = 3000, 48
+    dims, lo, cell = (37, 29, 13), (-1.2, -1.2, -1.2), (2.4 / 37, 2.4 / 29, 2.4 / 13)
+    w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=boundary, n=n)
+    pos, comp, fib = _spins(w, m, n, 9, plus_x=True)
+    q = _inject(n, iters, 10)
+    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+    sim.set_state(pos=pos, comp=comp, fiber=fib)
+    sim.moment_reset()
+    tr_o, tr_g = [], []
+    for k, wgt in enumerate((1.0, 0.0, -1.0)):
+        sl = q[k * 16:(k + 1) * 16]
+        tr_o.append(O.integrate(w, st, 16, rng=O.RNG_INJECT, inject=sl, weight=wgt, trace=True))
+        tr_g.append(sim.integrate_moment_injected(wgt, sl, trace=True))
+    g = sim.get_state()
+    mom, x0 = sim.get_moment()
+    tr_o, tr_g = np.concatenate(tr_o), np.concatenate(tr_g)
+    assert (tr_o[..., 2] > 0).sum() > 500
+    np.testing.assert_array_equal(tr_g, tr_o)
+    np.testing.assert_array_equal(g["pos"], st.pos)
+    np.testing.assert_array_equal(g["mag"], st.mag)
+    np.testing.assert_array_equal(g["comp"], st.comp)
+    np.testing.assert_array_equal(g["fiber"], st.fiber)
+    np.testing.assert_array_equal(mom, st.moment)
+    np.testing.assert_array_equal(x0, pos)
+    if boundary == 1:
+        assert (np.abs(st.pos) <= 1.0).all()
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_labels_match_oracle(ref_inputs, name, scale):
+    _say(f"case labels {name} scale={scale}")
+    m = _mesh(ref_inputs, name, scale)
+    n = 5000
+    for mode in (0, 1):
+        w, sub, sim = _pair(m, n=n, mode=mode)
+        rng = np.random.default_rng(3 + mode)
+        pos = rng.uniform(-0.95, 0.95, (n, 3)).astype(np.float32)
+        comp, fib, tries, amb = w.find_spins(pos, m["xmax"], m["ymax"], m["zmax"], plus_x=mode == 1)
+        sim.set_state(pos=pos)
+        gamb = sim.label(m["xmax"], m["ymax"], m["zmax"])
+        g = sim.get_state()
+        assert gamb == amb
+        np.testing.assert_array_equal(g["comp"], comp)
+        np.testing.assert_array_equal(g["fiber"], fib)
+        assert len(np.unique(comp)) >= 2
+
+
+def test_signal_matches_oracle(ref_inputs):
+    _say("case signal")
+    m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
+    n = 20000
+    w, sub, sim = _pair(m, n=n)
+    rng = np.random.default_rng(12)
+    pos = rng.uniform(-0.9, 0.9, (n, 3)).astype(np.float32)
+    comp = rng.integers(0, 4, n).astype(np.uint8)
+    mag = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    phase = rng.uniform(-40, 40, n).astype(np.float32)
+    sim.set_state(pos=pos, comp=comp, mag=mag, phase=phase)
+    np.testing.assert_allclose(sim.signal(), O.mr_signal(mag, phase, comp), rtol=1e-11, atol=1e-14)
+
+
+def test_moment_signal_matches_oracle():
+    _say("case moment-signal")
+    V = np.zeros((3, 3), np.float32) + 5
+    n = 30000
+    w, sub, sim = _pair({"vertices": V, "trgls": [[0, 1, 2]], "tri_info": [[0, 0, 0]]}, n=n,
+                        mode=1, grid="aabb", dims=(4, 4, 4), lo=(-1, -1, -1), cell=(0.5,) * 3)
+    sim.init_uniform(0.9)
+    sim.moment_reset()
+    sim.integrate_moment(1.0, 30)
+    sim.integrate_moment(-1.0, 30)
+    g = sim.get_state()
+    mom, x0 = sim.get_moment()
+    grads = np.array([[0.01, 0, 0], [0, 0.02, 0.01], [0, 0, 0]], np.float32)
+    np.testing.assert_allclose(sim.signal_moment(grads, PC),
+                               O.moment_signal(g["mag"], mom, g["comp"], grads, PC), rtol=1e-11,
+                               atol=1e-14)
+
+
+def test_mwc_stream_matches_oracle(ref_inputs):
+    _say("case mwc")
+    m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
+    n = 2000
+    w, sub, sim = _pair(m, n=n, rng=D.RNG_MWC)
+    pos, comp, fib = _spins(w, m, n, 5)
+    seeds = np.random.default_rng(6).integers(1, 2 ** 32, (n, 2), dtype=np.uint64).astype(np.uint32)
+    st = O.State(pos, comp=comp, fiber=fib, seed=seeds)
+    O.integrate(w, st, 1, rng=O.RNG_MWC)
+    sim.set_state(pos=pos, comp=comp, fiber=fib, seeds=seeds)
+    sim.integrate(np.zeros(3, np.float32), 0.0, 1)
+    g = sim.get_state(seeds=True)
+    np.testing.assert_array_equal(g["seeds"], st.seed)      # integer stream is exact
+    np.testing.assert_allclose(g["pos"], st.pos, rtol=1e-5, atol=2e-6)
+    assert np.mean(g["comp"] == st.comp) > 0.99
+
+
+def test_philox_stream_matches_oracle(ref_inputs):
+    _say("case philox")
+    m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
+    n = 4000
+    dims, lo, cell = (31, 31, 7), (-1.1, -1.1, -1.1), (2.2 / 31, 2.2 / 31, 2.2 / 7)
+    w, sub, sim = _pair(m, n=n, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, seed=0xABCDEF12345)
+    pos, comp, fib = _spins(w, m, n, 5, plus_x=True)
+    key = (0x12345 + (0xABCDEF12345 & 0) , 0)
+    key = (0xABCDEF12345 & 0xFFFFFFFF, 0xABCDEF12345 >> 32)
+    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+    sim.set_state(pos=pos, comp=comp, fiber=fib)
+    sim.moment_reset()
+    O.integrate(w, st, 20, rng=O.RNG_PHILOX, key=key, step0=0, weight=1.0)
+    sim.integrate_moment(1.0, 20)
+    g = sim.get_state()
+    # hardware log2/cos units vs libm: normals agree to ~1e-6 relative
+    close = np.all(np.abs(g["pos"] - st.pos) <= 1e-5 * np.maximum(1, np.abs(st.pos)), axis=1)
+    assert close.mean() > 0.995
+    assert np.mean(g["comp"] == st.comp) > 0.995
+
+
+def test_free_diffusion_signal_matches_exp_minus_bD():
+    """No membranes, PGSE 10/40/10 ms: S = exp(-b D) within 3 MC sigma."""
+    _say("case free-diffusion")
+    V = np.zeros((3, 3), np.float32) + 5
+    n = 200000
+    D_adc, space_scale = 2.0, 100.0
+    sd = np.float32(np.sqrt(2 * D_adc * DT) / space_scale)
+    w, sub, sim = _pair({"vertices": V, "trgls": [[0, 1, 2]], "tri_info": [[0, 0, 0]]}, n=n,
+                        mode=1, grid="aabb", dims=(4, 4, 4), lo=(-1.1,) * 3, cell=(0.55,) * 3,
+                        std=(sd,) * 4, t2=(np.inf,) * 4, perm=0.0, boundary=1)
+    sim.init_uniform(1.0)
+    sim.moment_reset()
+    L, B = 2000, 8000
+    sim.integrate_moment(1.0, L)
+    sim.integrate_moment(0.0, B - L)
+    sim.integrate_moment(-1.0, L)
+    sim.integrate_moment(0.0, 2000)
+    G = np.array([20.0, 40.0, 61.7])  # mT/m
+    grads = np.zeros((3, 3), np.float32)
+    grads[:, 0] = G * 1e-9 * space_scale
+    S = sim.signal_moment(grads, PC)[:, 0]
+    delta, Delta = 10.0, 40.0
+    b = (2 * np.pi * GAMMA * G * 1e-9 * delta) ** 2 * (Delta - delta / 3)   # diffusion.cpp:317
+    want = np.exp(-b * D_adc)
+    sigma = np.sqrt((1 - want ** 2) / (2 * n)) + 1e-4
+    assert np.all(np.abs(S - want) < 3 * sigma + 2e-3), (S, want)

@@ -1,0 +1,68 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared entry point, and the product's substrate preparation (triangle
+helpers, reference grid, AABB grid) is bit-identical to the oracle's."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dsim_amd import lib as D
+
+
+def test_library_exports_every_declared_symbol():
+    names = D.exported_names()
+    assert len(names) >= 25
+    L = C.CDLL(D.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert D.lib().dsim_abi_version() == 1
+
+
+def test_no_device_is_reported_cleanly():
+    if D.device_count() > 0:
+        pytest.skip("a device is visible")
+    sub = D.Substrate(np.zeros((3, 3), np.float32), [[0, 1, 2]], [[0, 0, 0]])
+    with pytest.raises(D.DsimError) as e:
+        D.Simulation(sub, 10, dt=0.005, std_dev=(0.01,) * 4)
+    assert e.value.code == -8
+
+
+def test_bad_mesh_rejected():
+    with pytest.raises(D.DsimError) as e:
+        D.Substrate(np.zeros((3, 3), np.float32), [[0, 1, 5]], [[0, 0, 0]])
+    assert e.value.code == -1
+    with pytest.raises(D.DsimError):
+        D.Substrate(np.zeros((3, 3), np.float32), [[0, 1, 2]], [[0, 7, 0]])
+
+
+MESHES = [("sim.fibers_1_r004_f09", 60.0), ("sim.fibers_1_r004_f09", 3.0),
+          ("sim.fibers_1_r0064_f05", 4.0), ("sim.triangles_curved", 1.0)]
+
+
+@pytest.mark.parametrize("name,scale", MESHES)
+def test_reference_grid_and_helpers_match_oracle(ref_inputs, name, scale):
+    m = O.parse_mesh(os.path.join(ref_inputs, name), scale)
+    sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="reference", n_cubes=50)
+    np.testing.assert_array_equal(sub.helpers(), O.tri_helpers(m["vertices"], m["trgls"]))
+    start, tri, maxc = sub.grid()
+    cnt, lst, omax = O.populate_cubes(m["vertices"], m["trgls"], 50)
+    assert maxc == omax
+    np.testing.assert_array_equal(np.diff(start), cnt)
+    for c in np.nonzero(cnt)[0]:
+        np.testing.assert_array_equal(tri[start[c]:start[c + 1]], lst[c, :cnt[c]])
+
+
+@pytest.mark.parametrize("name,scale", MESHES[1:])
+def test_aabb_grid_matches_oracle(ref_inputs, name, scale):
+    m = O.parse_mesh(os.path.join(ref_inputs, name), scale)
+    dims, lo, cell = (23, 19, 11), (-1.2, -1.1, -1.3), (2.4 / 23, 2.2 / 19, 2.6 / 11)
+    sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims, lo=lo,
+                      cell=cell, eps=1e-5)
+    start, tri, maxc = sub.grid()
+    cnt, lst, omax = O.populate_aabb(m["vertices"], m["trgls"], dims, lo, cell, 1e-5)
+    assert maxc == omax
+    np.testing.assert_array_equal(np.diff(start), cnt)
+    for c in np.nonzero(cnt)[0]:
+        np.testing.assert_array_equal(tri[start[c]:start[c + 1]], lst[c, :cnt[c]])
