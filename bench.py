@@ -1,0 +1,218 @@
+"""Benchmark: spin-steps/s at 1M spins on one MI355X.
+
+Workload (the at-scale configuration): 1e6 spins, 2000 z-aligned axons with
+radii ~ Gamma(k=3, mean 1 um) packed to an intra-axonal fraction of 0.7
+(voxel 107.9 um, periodic), 64-gon open prisms (286k triangles with the
+periodic images), permeability 0.01 um/ms, T2 70/90 ms, dt 5 us, PGSE
+delta/Delta/TE = 10/40/60 ms (12,000 steps), 64 directions on b = 1/2/3
+ms/um^2 shells plus b=0, fp32.  One bench step is one complete acquisition:
+every spin walks 12,000 steps once (the gradient-moment accumulation serves
+all 65 gradient lines) and the 65 signals are read out in FP64.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config scale|small]
+                       [--cell-um H] [--sort S] [--no-autotune] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from dsim_amd import autotune as AT  # noqa: E402
+from dsim_amd import lib as D  # noqa: E402
+from dsim_amd import substrate as S  # noqa: E402
+
+PEAK_FP32 = 157.3e12    # MI355X FP32 vector, FLOP/s
+GAMMA = 42576.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def schedule(cfg):
+    L = int(cfg.delta / cfg.dt + 0.5)
+    B = int(cfg.Delta / cfg.dt + 0.5)
+    R = int((cfg.TE - cfg.Delta - cfg.delta) / cfg.dt + 0.5)
+    # +G [0,L), 0 [L,B), -G [B,B+L), 0 [B+L, B+L+R)  (diffusion.cpp:564-585)
+    return [(1.0, L), (0.0, B - L), (-1.0, L), (0.0, R)]
+
+
+def flops_per_spin_step(n_tri, n_coll, n_ch=3):
+    """Work model: 50 + 7 n_ch + 45 n_tri + 21 n_coll FP32 operations (FMA = 2,
+    transcendental = 1) per spin-step: Box-Muller 24, segment 3, step 6, cell
+    9, wrap 6, T2 2, moment channels 7 each, Moller-Trumbore 45 per candidate
+    whose box overlaps the step's box, reflection 21 per membrane hit."""
+    return 50 + 7 * n_ch + 45 * n_tri + 21 * n_coll
+
+
+def cpu_baseline(env, sample_spins, target_s, threads):
+    """The oracle (plain-C restatement, same physical-mode semantics) on a
+    bounded sample of the same workload, timed on this host's cores."""
+    import oracle as O
+
+    grid = env["grid"]
+    w = O.World(env["vertices"], env["triangles"], env["tri_info"], grid="aabb", dims=grid["dims"],
+                lo=grid["lo"], cell=grid["cell"], eps=grid["eps"], std_dev=env["std_dev"],
+                t2=env["t2"], dt=env["cfg"].dt, permeability=env["permeability"], test=1,
+                traversal=1, transitions=1, boundary=1)
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(-1, 1, (sample_spins, 3)).astype(np.float32)
+    comp, fib, _, _ = w.find_spins(pos, env["xmax"], env["xmax"], env["xmax"], plus_x=True)
+    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+    stats = np.zeros(3, np.uint64)
+    cal = 100
+    t0 = time.perf_counter()
+    O.integrate(w, st, cal, rng=O.RNG_PHILOX, key=(5, 6), step0=0, weight=1.0, n_threads=threads,
+                stats=stats)
+    dt0 = time.perf_counter() - t0
+    steps = int(max(cal, min(12000, cal * target_s / max(dt0, 1e-6))))
+    t0 = time.perf_counter()
+    O.integrate(w, st, steps, rng=O.RNG_PHILOX, key=(5, 6), step0=cal, weight=1.0,
+                n_threads=threads)
+    el = time.perf_counter() - t0
+    n_steps, n_cand, n_hit = (int(x) for x in stats)
+    return dict(value=sample_spins * steps / el, unit="spin-steps/s", cores=threads, kind="port",
+                sample=f"{sample_spins} spins x {steps} steps of the same substrate and physics "
+                       f"(oracle, {threads} OpenMP threads, {el:.1f} s)",
+                n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="scale", choices=["scale", "small"])
+    ap.add_argument("--spins", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cell-um", type=float, default=0.0)
+    ap.add_argument("--sort", type=int, default=-1)
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the simulate path is data-parallel over spins and "
+              f"shards by spin id with one FP64 signal all-reduce (see DESIGN.md); "
+              f"--gpus {args.gpus} is not supported in this build.", file=sys.stderr)
+        sys.exit(2)
+    if D.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    cfg = S.SCALE if args.config == "scale" else S.SMALL
+    n = args.spins or cfg.n_spins
+    t0 = time.perf_counter()
+    axons = S.pack(cfg)
+    log(f"packed {cfg.n_axons} axons: voxel {axons.voxel:.2f} um, intra fraction "
+        f"{axons.intra_fraction:.3f} ({time.perf_counter() - t0:.1f} s)")
+
+    tuned = None
+    cell_um, sort_interval = args.cell_um or 0.25, 128 if args.sort < 0 else args.sort
+    if not args.no_autotune and not args.cell_um:
+        t0 = time.perf_counter()
+        best, allr = AT.tune(cfg, axons, n, sorts=(0, 128) if args.sort < 0 else (args.sort,),
+                             log=log)
+        cell_um, sort_interval = best["cell_um"], best["sort_interval"]
+        tuned = dict(cell_um=cell_um, sort_interval=sort_interval,
+                     candidates=[{k: r[k] for k in ("cell_um", "sort_interval", "rate")} for r in allr],
+                     seconds=round(time.perf_counter() - t0, 1))
+        log(f"autotune picked cell {cell_um} um, sort every {sort_interval} steps")
+
+    env = S.build(cfg, cell_um=cell_um, axons=axons)
+    g = env["grid"]
+    sub, sim = AT.make_sim(env, n, sort_interval=sort_interval)
+    _, _, maxc = sub.grid()
+    t0 = time.perf_counter()
+    sim.init_uniform(1.0)
+    amb = sim.label(env["xmax"], env["xmax"], env["xmax"])
+    st = sim.get_state()
+    frac_intra = float(np.mean(st["comp"] == 1))
+    log(f"{len(env['triangles'])} triangles, grid {g['dims']} cell {cell_um} um; labelled {n} "
+        f"spins in {time.perf_counter() - t0:.2f} s: intra {frac_intra:.3f}, ambiguous {amb}")
+
+    prot = env["protocol"]
+    grads = (prot[:, 3:6].astype(np.float64) * 1e-9 * env["space_scale"]).astype(np.float32)
+    pc = np.float32(2 * np.pi * GAMMA * cfg.dt)
+    sched = schedule(cfg)
+    steps_per_acq = sum(k for _, k in sched)
+
+    def acquisition():
+        sim.moment_reset()
+        for wgt, k in sched:
+            sim.integrate_moment(wgt, k)
+        return sim.signal_moment(grads, pc)
+
+    for _ in range(args.warmup):
+        acquisition()
+    sim.sync()
+    sim.step_timing_reset()
+    t0 = time.perf_counter()
+    sig = None
+    for _ in range(args.steps):
+        sig = acquisition()
+    sim.sync()
+    wall = time.perf_counter() - t0
+    step_ms, launches = sim.step_timing()
+    spin_steps = n * steps_per_acq * args.steps
+    value = spin_steps / wall
+    log(f"{args.steps} acquisitions in {wall:.3f} s; step kernels {step_ms / 1e3:.3f} s over "
+        f"{launches} launches; b=0 signal {sig[0, 0]:.5f}, b=1 mean {sig[1:22, 0].mean():.5f}")
+
+    cpu = None
+    n_tri, n_coll = None, None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(env, 20000, args.cpu_seconds, threads)
+        n_tri, n_coll = cpu.pop("n_tri"), cpu.pop("n_coll")
+        log(f"cpu baseline {cpu['value']:.3e} spin-steps/s ({cpu['sample']}); work model "
+            f"n_tri {n_tri:.3f}, n_coll {n_coll:.4f}")
+    roof = None
+    if n_tri is not None:
+        F = flops_per_spin_step(n_tri, n_coll)
+        avg_launch_s = step_ms / 1e3 / max(1, launches)
+        achieved = F * n * steps_per_acq * args.steps / (step_ms / 1e3)
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                "flops_per_spin_step": F, "n_tri": n_tri, "n_coll": n_coll,
+                "avg_launch_ms": avg_launch_s * 1e3,
+                "spin_steps_per_launch": n * steps_per_acq * args.steps / max(1, launches)}
+    out = {
+        "metric": "spin-steps/s at 1M spins",
+        "value": value,
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg.name, "spins": n, "axons": cfg.n_axons,
+                   "triangles": int(len(env["triangles"])), "voxel_um": round(axons.voxel, 2),
+                   "intra_fraction": round(axons.intra_fraction, 4),
+                   "steps_per_acquisition": steps_per_acq, "gradient_lines": int(len(prot)),
+                   "dt_ms": cfg.dt, "permeability_per_hit": env["permeability"],
+                   "grid": list(g["dims"]), "cell_um": cell_um, "sort_interval": sort_interval,
+                   "max_tris_per_cell": maxc, "parallelism": "none (1 GPU)"},
+        "reference_equivalent_value": value * len(prot),
+        "autotune": tuned,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "host": platform.processor() or platform.machine(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+"""Launch-shape autotuner for the step kernel.
+
+The grid cell edge sets how many candidate triangles a step tests (small
+cells: few candidates but more cells per step; large cells: long lists), and
+the spatial-sort interval sets how coherent a wavefront's cells stay.  Both
+depend on the substrate, so they are chosen by timing short runs of the real
+workload on the device: every candidate builds its grid, walks all spins for
+a few hundred steps, and the fastest (steps per second from HIP events on the
+step kernels) wins.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import lib as D
+from . import substrate as S
+
+
+def make_sim(env, n, *, sort_interval, steps_per_launch=0, seed=None):
+    g = env["grid"]
+    cfg = env["cfg"]
+    sub = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
+                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+    sim = D.Simulation(sub, n, dt=cfg.dt, std_dev=env["std_dev"], t2=env["t2"],
+                       permeability=env["permeability"], mode=D.MODE_PHYSICAL,
+                       boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX,
+                       seed=cfg.seed if seed is None else seed, sort_interval=sort_interval,
+                       steps_per_launch=steps_per_launch)
+    return sub, sim
+
+
+def time_steps(sim, steps, warm=64):
+    sim.moment_reset()
+    sim.integrate_moment(1.0, warm)
+    sim.sync()
+    sim.step_timing_reset()
+    sim.integrate_moment(1.0, steps)
+    sim.sync()
+    ms, _ = sim.step_timing()
+    return ms / 1e3
+
+
+def tune(cfg, axons, n, *, cells_um=(0.2, 0.25, 0.33, 0.5, 1.0), sorts=(0, 128),
+         steps=256, log=None):
+    """Returns (best dict, all results).  Each result: cell_um, sort_interval,
+    spin-steps/s, grid dims."""
+    results = []
+    for h in cells_um:
+        env = S.build(cfg, cell_um=h, axons=axons)
+        for srt in sorts:
+            t0 = time.perf_counter()
+            sub, sim = make_sim(env, n, sort_interval=srt)
+            sim.init_uniform(1.0)
+            sim.label(env["xmax"], env["xmax"], env["xmax"])
+            sec = time_steps(sim, steps)
+            rate = n * steps / sec
+            r = dict(cell_um=h, sort_interval=srt, rate=rate, grid=list(env["grid"]["dims"]),
+                     setup_s=time.perf_counter() - t0 - sec)
+            results.append(r)
+            if log:
+                log(f"autotune cell {h:.3f} um sort {srt}: {rate:.3e} spin-steps/s "
+                    f"(grid {r['grid']})")
+            sim.close()
+            sub.close()
+    best = max(results, key=lambda r: r["rate"])
+    return best, results

@@ -33,8 +33,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFOR
              "-Wall", "-Wno-misleading-indentation", "-Wno-unused-result", "-pthread"]
 
 HIP_SOURCES = ["kernels.hip"]
-CXX_SOURCES = ["host.cpp"]
-MAIN_SOURCES: list = []
+CXX_SOURCES = ["host.cpp", "sim.cpp", "formats.cpp"]
+MAIN_SOURCES = ["spin_system.cpp", "dsim_main.cpp"]
 
 
 def _run(cmd):

@@ -23,6 +23,7 @@ struct StepParams {
     const float4 *tri_mt;
     const uint32_t *cell_start;
     const uint32_t *cell_tri;
+    const float4 *cell_plane;  // plane (n, n.v0) of cell_tri[k], cell-major
     const uint8_t *fiber_myelin;
     uint32_t n_fibers;
     uint32_t dims[3];
@@ -41,6 +42,7 @@ struct StepParams {
     uint32_t n;
     float *x, *y, *z, *mag, *phase;
     uint32_t *label;  // compartment | fibre << 8
+    const uint32_t *spin_id;  // identity of the spin held by each slot (Philox counter)
     uint32_t *seed;   // MWC, 2 per spin
     float *disp;      // [3][n] unwrapped displacement
     float *mom;       // [3][n] moment
@@ -56,6 +58,8 @@ struct StepParams {
 
 struct LabelParams {
     const float4 *tri_ref;
+    const float4 *tri_mt;
+    uint32_t use_mt;  // 1: Moller-Trumbore crossing test (physical mode)
     const float4 *tri_aabb;
     uint32_t n_tri;
     uint32_t n;
@@ -81,11 +85,26 @@ hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, 
                                 double *out, hipStream_t s);
 hipError_t launch_label(const LabelParams &p, hipStream_t s);
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
-                               uint32_t *label, float start_box, uint32_t key0, uint32_t key1,
-                               hipStream_t s);
+                               uint32_t *label, uint32_t *id, float start_box, uint32_t key0,
+                               uint32_t key1, hipStream_t s);
 hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
-                               float *x0, float *disp, float *mom, float *momc, hipStream_t s);
+                               float *x0, float *disp, float *mom, float *momc, float *mag,
+                               hipStream_t s);
 
 constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout kernels
+
+// Spatial reordering of spins: Morton key of each slot's cell, radix sort of
+// (key, slot), then a gather of every per-spin field into its new slot.
+constexpr uint32_t kMaxFields = 24;
+struct Fields {
+    uint32_t *src[kMaxFields];
+    uint32_t *dst[kMaxFields];
+    uint32_t count;
+};
+hipError_t sort_temp_bytes(uint32_t n, size_t *bytes);
+hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *keys_out,
+                            uint32_t *vals_in, uint32_t *vals_out, void *temp, size_t temp_bytes,
+                            hipStream_t s);
+hipError_t launch_gather(uint32_t n, const uint32_t *perm, const Fields &f, hipStream_t s);
 
 }  // namespace dsim

@@ -17,10 +17,9 @@
 
 #include "../../include/dsim.h"
 #include "dsim_internal.h"
+#include "host_common.h"
 
-using dsim::kNone;
-
-namespace {
+namespace dsim_host {
 
 thread_local std::string g_err;
 
@@ -34,12 +33,13 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(DSIM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
-    } while (0)
+}  // namespace dsim_host
+
+using dsim_host::DevBuf;
+using dsim_host::fail;
+using dsim_host::upload;
+
+namespace {
 
 struct V3 {
     float x, y, z;
@@ -49,36 +49,6 @@ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
 inline float dotv(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    int alloc(size_t count) {
-        n = count;
-        if (count == 0) return DSIM_OK;
-        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(DSIM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T),
-                        hipGetErrorString(e));
-        }
-        return DSIM_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-template <class T>
-int upload(DevBuf<T> &d, const std::vector<T> &h) {
-    int rc = d.alloc(h.size());
-    if (rc) return rc;
-    if (!h.empty()) HIP_TRY(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-    return DSIM_OK;
-}
 
 inline float bits_to_float(uint32_t u) {
     float f;
@@ -222,26 +192,7 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
 // Substrate
 // ---------------------------------------------------------------------------
 
-struct dsim_substrate {
-    uint32_t n_tri = 0, n_vert = 0, n_fibers = 0;
-    std::vector<float> helpers;           // [n_tri*12]
-    std::vector<uint32_t> cell_start, cell_tri;
-    uint32_t n_cells = 0, max_per_cell = 0;
-    uint32_t kind = 0;
-    uint32_t dims[3] = {0, 0, 0};
-    float lo[3] = {0, 0, 0}, cell[3] = {0, 0, 0}, inv_cell[3] = {0, 0, 0};
-    DevBuf<float4> tri_ref, tri_mt, tri_aabb;
-    DevBuf<uint32_t> d_cell_start, d_cell_tri;
-    DevBuf<uint8_t> fiber_myelin;
-    int refs = 0;
-    bool on_device = false;
-    ~dsim_substrate() {
-        tri_ref.release(); tri_mt.release(); tri_aabb.release();
-        d_cell_start.release(); d_cell_tri.release(); fiber_myelin.release();
-    }
-};
-
-extern "C" const char *dsim_last_error(void) { return g_err.c_str(); }
+extern "C" const char *dsim_last_error(void) { return dsim_host::g_err.c_str(); }
 extern "C" int dsim_abi_version(void) { return DSIM_ABI_VERSION; }
 
 extern "C" int dsim_device_count(int *n) {
@@ -415,7 +366,11 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     }
     std::vector<uint32_t> ct = S->cell_tri;
     if (ct.empty()) ct.push_back(0);  // keep a valid pointer for empty grids
-    if ((rc = upload(S->d_cell_tri, ct))) {
+    // plane of every grid entry, stored in cell order so the prefilter of a
+    // cell's candidates reads contiguous memory
+    std::vector<float4> cp(ct.size());
+    for (size_t k = 0; k < ct.size(); ++k) cp[k] = S->cell_tri.empty() ? float4{} : mt[(size_t)ct[k] * 4];
+    if ((rc = upload(S->d_cell_tri, ct)) || (rc = upload(S->cell_plane, cp))) {
         delete S;
         return rc;
     }
@@ -455,7 +410,7 @@ extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helper
 
 extern "C" int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(DSIM_ERR_ARG, "NULL argument");
-    *bytes = (uint64_t)s->tri_ref.n * 16 + s->tri_mt.n * 16 + s->tri_aabb.n * 16 +
+    *bytes = (uint64_t)s->tri_ref.n * 16 + s->tri_mt.n * 16 + s->tri_aabb.n * 16 + s->cell_plane.n * 16 +
              s->d_cell_start.n * 4 + s->d_cell_tri.n * 4 + s->fiber_myelin.n;
     return DSIM_OK;
 }
@@ -463,473 +418,3 @@ extern "C" int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes) {
 // ---------------------------------------------------------------------------
 // Simulation handle
 // ---------------------------------------------------------------------------
-
-struct dsim_sim {
-    dsim_substrate *sub = nullptr;
-    dsim_params prm{};
-    uint32_t n = 0;
-    uint64_t step = 0;
-    hipStream_t stream = nullptr;
-    DevBuf<float> x, y, z, mag, phase, disp, mom, momc, x0;
-    DevBuf<uint32_t> label, seed, flags, tries;
-    DevBuf<double> partial, out;
-    DevBuf<float> grads;
-    DevBuf<float4> inject;
-    DevBuf<uint32_t> trace;
-    float wsum = 0.0f;
-    bool moment_ready = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    double ms_done = 0.0;
-    uint64_t launches = 0;
-    ~dsim_sim() {
-        for (auto &e : ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-        x.release(); y.release(); z.release(); mag.release(); phase.release(); disp.release();
-        mom.release(); momc.release(); x0.release(); label.release(); seed.release();
-        flags.release(); tries.release(); partial.release(); out.release(); grads.release();
-        inject.release(); trace.release();
-        if (stream) (void)hipStreamDestroy(stream);
-        if (sub) sub->refs--;
-    }
-};
-
-static int check_sim(dsim_sim *s) {
-    if (!s) return fail(DSIM_ERR_ARG, "NULL sim");
-    return DSIM_OK;
-}
-
-extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32_t n,
-                               dsim_sim **out) {
-    if (!sub || !p || !out) return fail(DSIM_ERR_ARG, "NULL argument");
-    *out = nullptr;
-    if (!sub->on_device) return fail(DSIM_ERR_NODEV, "substrate was created without a HIP device");
-    if (p->mode > 1 || p->boundary > 1 || (p->rng != DSIM_RNG_MWC && p->rng != DSIM_RNG_PHILOX))
-        return fail(DSIM_ERR_ARG, "bad mode/boundary/rng");
-    if (p->mode == DSIM_MODE_REFERENCE && p->boundary != DSIM_BOUNDARY_CLAMP)
-        return fail(DSIM_ERR_ARG, "the reference mode clamps (spinKernelCpu.cpp:469-474)");
-    if (!(p->dt > 0.0f) || !(p->permeability >= 0.0f && p->permeability <= 1.0f))
-        return fail(DSIM_ERR_ARG, "dt must be > 0 and permeability in [0,1]");
-    auto *S = new dsim_sim();
-    S->sub = sub;
-    sub->refs++;
-    S->prm = *p;
-    S->n = n;
-    hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete S;
-        return fail(DSIM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-    }
-    int rc;
-    size_t m = std::max<uint32_t>(n, 1);
-    if ((rc = S->x.alloc(m)) || (rc = S->y.alloc(m)) || (rc = S->z.alloc(m)) ||
-        (rc = S->mag.alloc(m)) || (rc = S->phase.alloc(m)) || (rc = S->label.alloc(m)) ||
-        (rc = S->flags.alloc(4)) || (rc = S->partial.alloc((size_t)dsim::kSignalBlocks * 15)) ||
-        (rc = S->out.alloc(15)) || (rc = S->tries.alloc(m))) {
-        delete S;
-        return rc;
-    }
-    if (p->rng == DSIM_RNG_MWC && (rc = S->seed.alloc(2 * m))) {
-        delete S;
-        return rc;
-    }
-    const std::pair<void *, size_t> zero[] = {
-        {S->flags.p, 16},    {S->mag.p, m * 4}, {S->phase.p, m * 4}, {S->x.p, m * 4},
-        {S->y.p, m * 4},     {S->z.p, m * 4},   {S->label.p, m * 4},
-        {S->seed.p, S->seed.p ? 2 * m * 4 : 0}};
-    for (const auto &zb : zero) {
-        if (!zb.first || !zb.second) continue;
-        e = hipMemsetAsync(zb.first, 0, zb.second, S->stream);
-        if (e != hipSuccess) {
-            delete S;
-            return fail(DSIM_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
-        }
-    }
-    e = hipStreamSynchronize(S->stream);
-    if (e != hipSuccess) {
-        delete S;
-        return fail(DSIM_ERR_HIP, "init: %s", hipGetErrorString(e));
-    }
-    *out = S;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_sim_destroy(dsim_sim *s) {
-    if (!s) return DSIM_OK;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    delete s;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_set_state_aos(dsim_sim *s, const float *pos, const dsim_spin_data *spins,
-                                  const uint32_t *seeds) {
-    if (int rc = check_sim(s)) return rc;
-    const uint32_t n = s->n;
-    if (n == 0) return DSIM_OK;
-    HIP_TRY(hipStreamSynchronize(s->stream));  // the copies below use the null stream
-    if (pos) {
-        std::vector<float> a(n), b(n), c(n);
-        for (uint32_t i = 0; i < n; ++i) { a[i] = pos[3 * i]; b[i] = pos[3 * i + 1]; c[i] = pos[3 * i + 2]; }
-        HIP_TRY(hipMemcpy(s->x.p, a.data(), n * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(s->y.p, b.data(), n * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(s->z.p, c.data(), n * 4, hipMemcpyHostToDevice));
-    }
-    if (spins) {
-        std::vector<float> m(n), ph(n);
-        std::vector<uint32_t> lab(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            if (spins[i].compartmentType > 3)
-                return fail(DSIM_ERR_ARG, "spin %u compartment %u > 3", i, spins[i].compartmentType);
-            m[i] = spins[i].signalMagnitude;
-            ph[i] = spins[i].signalPhase;
-            lab[i] = (uint32_t)spins[i].compartmentType | ((uint32_t)spins[i].insideFiber << 8);
-        }
-        HIP_TRY(hipMemcpy(s->mag.p, m.data(), n * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(s->phase.p, ph.data(), n * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(s->label.p, lab.data(), n * 4, hipMemcpyHostToDevice));
-    }
-    if (seeds) {
-        if (!s->seed.p) return fail(DSIM_ERR_STATE, "seeds given but rng is not MWC");
-        HIP_TRY(hipMemcpy(s->seed.p, seeds, (size_t)n * 8, hipMemcpyHostToDevice));
-    }
-    s->moment_ready = false;
-    return DSIM_OK;
-}
-
-static int sync_and_check(dsim_sim *s) {
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    uint32_t f[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpy(f, s->flags.p, 16, hipMemcpyDeviceToHost));
-    if (f[0] & 1u) {
-        HIP_TRY(hipMemset(s->flags.p, 0, 16));
-        return fail(DSIM_ERR_OVERFLOW, "a spin exceeded 4096 collisions in one step");
-    }
-    return DSIM_OK;
-}
-
-extern "C" int dsim_get_state_aos(dsim_sim *s, float *pos, dsim_spin_data *spins,
-                                  uint32_t *seeds) {
-    if (int rc = check_sim(s)) return rc;
-    if (int rc = sync_and_check(s)) return rc;
-    const uint32_t n = s->n;
-    if (n == 0) return DSIM_OK;
-    if (pos) {
-        std::vector<float> a(n), b(n), c(n);
-        HIP_TRY(hipMemcpy(a.data(), s->x.p, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(b.data(), s->y.p, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(c.data(), s->z.p, n * 4, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i) { pos[3 * i] = a[i]; pos[3 * i + 1] = b[i]; pos[3 * i + 2] = c[i]; }
-    }
-    if (spins) {
-        std::vector<float> m(n), ph(n);
-        std::vector<uint32_t> lab(n);
-        HIP_TRY(hipMemcpy(m.data(), s->mag.p, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(ph.data(), s->phase.p, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(lab.data(), s->label.p, n * 4, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i) {
-            memset(&spins[i], 0, sizeof(dsim_spin_data));
-            spins[i].signalMagnitude = m[i];
-            spins[i].signalPhase = ph[i];
-            spins[i].compartmentType = (uint8_t)(lab[i] & 0xFFu);
-            spins[i].insideFiber = (uint16_t)(lab[i] >> 8);
-        }
-    }
-    if (seeds) {
-        if (!s->seed.p) return fail(DSIM_ERR_STATE, "rng is not MWC");
-        HIP_TRY(hipMemcpy(seeds, s->seed.p, (size_t)n * 8, hipMemcpyDeviceToHost));
-    }
-    return DSIM_OK;
-}
-
-extern "C" int dsim_set_step(dsim_sim *s, uint64_t step) {
-    if (int rc = check_sim(s)) return rc;
-    s->step = step;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_get_step(dsim_sim *s, uint64_t *step) {
-    if (int rc = check_sim(s)) return rc;
-    if (!step) return fail(DSIM_ERR_ARG, "NULL step");
-    *step = s->step;
-    return DSIM_OK;
-}
-
-static dsim::StepParams step_params(dsim_sim *s) {
-    dsim::StepParams P{};
-    dsim_substrate *b = s->sub;
-    P.tri_ref = b->tri_ref.p;
-    P.tri_mt = b->tri_mt.p;
-    P.cell_start = b->d_cell_start.p;
-    P.cell_tri = b->d_cell_tri.p;
-    P.fiber_myelin = b->fiber_myelin.p;
-    P.n_fibers = b->n_fibers;
-    for (int k = 0; k < 3; ++k) {
-        P.dims[k] = b->dims[k]; P.lo[k] = b->lo[k]; P.cell[k] = b->cell[k]; P.inv_cell[k] = b->inv_cell[k];
-    }
-    for (int c = 0; c < 4; ++c) {
-        P.std_dev[c] = s->prm.std_dev[c];
-        P.t2[c] = s->prm.t2[c];
-        P.decay[c] = s->prm.dt / s->prm.t2[c];
-    }
-    P.dt = s->prm.dt;
-    P.perm = s->prm.permeability;
-    P.reflection_type = s->prm.reflection_type;
-    P.clamp_hi = 1.0f - s->prm.spin_radius;
-    P.clamp_lo = -1.0f + s->prm.spin_radius;
-    P.n = s->n;
-    P.x = s->x.p; P.y = s->y.p; P.z = s->z.p; P.mag = s->mag.p; P.phase = s->phase.p;
-    P.label = s->label.p;
-    P.seed = s->seed.p;
-    P.disp = s->disp.p; P.mom = s->mom.p; P.momc = s->momc.p;
-    P.key0 = (uint32_t)s->prm.seed;
-    P.key1 = (uint32_t)(s->prm.seed >> 32);
-    P.flags = s->flags.p;
-    return P;
-}
-
-// Largest step count per launch: keeps single kernels short.
-static const uint32_t kMaxStepsPerLaunch = 256;
-
-static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, bool moment,
-                     const float *n3u, uint32_t *trace) {
-    const uint32_t n = s->n;
-    if (n3u) {
-        if ((uint64_t)iters * n * 16 > (1ull << 32))
-            return fail(DSIM_ERR_ARG, "injected sequence too large (%u x %u)", iters, n);
-        if (s->inject.n < (size_t)iters * n) {
-            s->inject.release();
-            if (int rc = s->inject.alloc((size_t)iters * n)) return rc;
-        }
-        HIP_TRY(hipMemcpyAsync(s->inject.p, n3u, (size_t)iters * n * 16, hipMemcpyHostToDevice,
-                               s->stream));
-    }
-    if (trace) {
-        if ((uint64_t)iters * n * 20 > (1ull << 32))
-            return fail(DSIM_ERR_ARG, "trace too large (%u x %u)", iters, n);
-        if (s->trace.n < (size_t)iters * n * 5) {
-            s->trace.release();
-            if (int rc = s->trace.alloc((size_t)iters * n * 5)) return rc;
-        }
-    }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, s->stream));
-    uint32_t done = 0;
-    while (done < iters) {
-        uint32_t k = std::min(kMaxStepsPerLaunch, iters - done);
-        P.step0 = s->step + done;
-        P.inject = n3u ? s->inject.p + (size_t)done * n : nullptr;
-        P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
-        hipError_t e = dsim::launch_step(P, k, (int)s->prm.mode,
-                                         s->prm.boundary == DSIM_BOUNDARY_PERIODIC, rng,
-                                         trace != nullptr, moment, s->stream);
-        if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
-        done += k;
-        s->launches++;
-    }
-    HIP_TRY(hipEventRecord(e1, s->stream));
-    s->ev.emplace_back(e0, e1);
-    s->step += iters;
-    if (trace) {
-        HIP_TRY(hipMemcpyAsync(trace, s->trace.p, (size_t)iters * n * 20, hipMemcpyDeviceToHost,
-                               s->stream));
-        if (int rc = sync_and_check(s)) return rc;
-    }
-    return DSIM_OK;
-}
-
-static int rng_kind(dsim_sim *s, const float *n3u) {
-    if (n3u) return dsim::kRngInject;
-    return s->prm.rng == DSIM_RNG_MWC ? dsim::kRngMwc : dsim::kRngPhilox;
-}
-
-extern "C" int dsim_integrate(dsim_sim *s, const float g[3], float pc, uint32_t iters) {
-    return dsim_integrate_injected(s, g, pc, nullptr, iters, nullptr);
-}
-
-extern "C" int dsim_integrate_injected(dsim_sim *s, const float g[3], float pc, const float *n3u,
-                                       uint32_t iters, uint32_t *trace) {
-    if (int rc = check_sim(s)) return rc;
-    if (!g) return fail(DSIM_ERR_ARG, "NULL gradient");
-    dsim::StepParams P = step_params(s);
-    P.gx = g[0]; P.gy = g[1]; P.gz = g[2]; P.pc = pc;
-    return run_steps(s, P, iters, rng_kind(s, n3u), false, n3u, trace);
-}
-
-extern "C" int dsim_signal(dsim_sim *s, double out[5]) {
-    if (int rc = check_sim(s)) return rc;
-    if (!out) return fail(DSIM_ERR_ARG, "NULL out");
-    hipError_t e = dsim::launch_signal(s->n, s->mag.p, s->phase.p, s->label.p, s->partial.p,
-                                       dsim::kSignalBlocks, s->out.p, s->stream);
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "signal launch: %s", hipGetErrorString(e));
-    double v[15];
-    HIP_TRY(hipMemcpyAsync(v, s->out.p, sizeof v, hipMemcpyDeviceToHost, s->stream));
-    if (int rc = sync_and_check(s)) return rc;
-    for (int k = 0; k < 5; ++k)
-        out[k] = v[10 + k] == 0.0 ? 0.0 : std::sqrt(v[k] * v[k] + v[5 + k] * v[5 + k]) / v[10 + k];
-    return DSIM_OK;
-}
-
-static int ensure_moment(dsim_sim *s) {
-    size_t m = 3 * (size_t)std::max<uint32_t>(s->n, 1);
-    int rc;
-    if (!s->disp.p && ((rc = s->disp.alloc(m)) || (rc = s->mom.alloc(m)) ||
-                       (rc = s->momc.alloc(m)) || (rc = s->x0.alloc(m))))
-        return rc;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_moment_reset(dsim_sim *s) {
-    if (int rc = check_sim(s)) return rc;
-    if (int rc = ensure_moment(s)) return rc;
-    hipError_t e = dsim::launch_moment_reset(s->n, s->x.p, s->y.p, s->z.p, s->x0.p, s->disp.p,
-                                             s->mom.p, s->momc.p, s->stream);
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "moment reset: %s", hipGetErrorString(e));
-    s->wsum = 0.0f;
-    s->moment_ready = true;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_integrate_moment(dsim_sim *s, float w, uint32_t iters) {
-    return dsim_integrate_moment_injected(s, w, nullptr, iters, nullptr);
-}
-
-extern "C" int dsim_integrate_moment_injected(dsim_sim *s, float w, const float *n3u,
-                                              uint32_t iters, uint32_t *trace) {
-    if (int rc = check_sim(s)) return rc;
-    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "call dsim_moment_reset first");
-    dsim::StepParams P = step_params(s);
-    P.weight = w;
-    int rc = run_steps(s, P, iters, rng_kind(s, n3u), true, n3u, trace);
-    if (rc == DSIM_OK) s->wsum += w * (float)iters;
-    return rc;
-}
-
-extern "C" int dsim_signal_moment(dsim_sim *s, uint32_t ndir, const float *g, float pc,
-                                  double *out) {
-    if (int rc = check_sim(s)) return rc;
-    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "no moments accumulated");
-    if (ndir == 0) return DSIM_OK;
-    if (!g || !out) return fail(DSIM_ERR_ARG, "NULL argument");
-    const uint32_t nb = 256;
-    if (s->grads.n < (size_t)ndir * 3) {
-        s->grads.release();
-        if (int rc = s->grads.alloc((size_t)ndir * 3)) return rc;
-    }
-    if (s->partial.n < (size_t)ndir * nb * 15) {
-        s->partial.release();
-        if (int rc = s->partial.alloc((size_t)std::max<size_t>(ndir * (size_t)nb, dsim::kSignalBlocks) * 15)) return rc;
-    }
-    DevBuf<double> o;
-    if (int rc = o.alloc((size_t)ndir * 15)) return rc;
-    HIP_TRY(hipMemcpyAsync(s->grads.p, g, (size_t)ndir * 12, hipMemcpyHostToDevice, s->stream));
-    hipError_t e = dsim::launch_moment_signal(s->n, s->mag.p, s->mom.p, s->x0.p, s->wsum,
-                                              s->label.p, ndir, s->grads.p, pc, s->partial.p, nb,
-                                              o.p, s->stream);
-    if (e != hipSuccess) { o.release(); return fail(DSIM_ERR_HIP, "moment signal: %s", hipGetErrorString(e)); }
-    std::vector<double> v((size_t)ndir * 15);
-    e = hipMemcpyAsync(v.data(), o.p, v.size() * 8, hipMemcpyDeviceToHost, s->stream);
-    int rc = sync_and_check(s);
-    o.release();
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "copy: %s", hipGetErrorString(e));
-    if (rc) return rc;
-    for (uint32_t j = 0; j < ndir; ++j) {
-        const double *q = &v[(size_t)j * 15];
-        for (int k = 0; k < 5; ++k)
-            out[j * 5 + k] = q[10 + k] == 0.0 ? 0.0 : std::sqrt(q[k] * q[k] + q[5 + k] * q[5 + k]) / q[10 + k];
-    }
-    return DSIM_OK;
-}
-
-extern "C" int dsim_get_moment(dsim_sim *s, float *moment, float *x0) {
-    if (int rc = check_sim(s)) return rc;
-    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "no moments accumulated");
-    if (int rc = sync_and_check(s)) return rc;
-    const uint32_t n = s->n;
-    std::vector<float> t(3 * (size_t)n);
-    if (moment) {
-        HIP_TRY(hipMemcpy(t.data(), s->mom.p, t.size() * 4, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) moment[3 * i + k] = t[(size_t)k * n + i];
-    }
-    if (x0) {
-        HIP_TRY(hipMemcpy(t.data(), s->x0.p, t.size() * 4, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) x0[3 * i + k] = t[(size_t)k * n + i];
-    }
-    return DSIM_OK;
-}
-
-extern "C" int dsim_init_spins_uniform(dsim_sim *s, float box) {
-    if (int rc = check_sim(s)) return rc;
-    hipError_t e = dsim::launch_init_uniform(s->n, s->x.p, s->y.p, s->z.p, s->mag.p, s->phase.p,
-                                             s->label.p, box, (uint32_t)s->prm.seed,
-                                             (uint32_t)(s->prm.seed >> 32), s->stream);
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
-    s->moment_ready = false;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
-                                uint32_t max_tries, uint32_t *n_ambiguous) {
-    if (int rc = check_sim(s)) return rc;
-    dsim::LabelParams L{};
-    L.tri_ref = s->sub->tri_ref.p;
-    L.tri_aabb = s->sub->tri_aabb.p;
-    L.n_tri = s->sub->n_tri;
-    L.n = s->n;
-    L.x = s->x.p; L.y = s->y.p; L.z = s->z.p;
-    L.label = s->label.p;
-    L.tries = s->tries.p;
-    L.xmax = xmax; L.ymax = ymax; L.zmax = zmax;
-    L.max_tries = max_tries;
-    L.plus_x = s->prm.mode == DSIM_MODE_PHYSICAL ? 1u : 0u;
-    L.flags = s->flags.p;
-    hipError_t e = dsim::launch_label(L, s->stream);
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "label launch: %s", hipGetErrorString(e));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    uint32_t f[4];
-    HIP_TRY(hipMemcpy(f, s->flags.p, 16, hipMemcpyDeviceToHost));
-    if (n_ambiguous) {
-        std::vector<uint32_t> t(s->n);
-        if (s->n) HIP_TRY(hipMemcpy(t.data(), s->tries.p, (size_t)s->n * 4, hipMemcpyDeviceToHost));
-        uint32_t a = 0;
-        for (uint32_t v : t) a += v > max_tries;
-        *n_ambiguous = a;
-    }
-    if (f[0] & 2u) {
-        HIP_TRY(hipMemset(s->flags.p, 0, 16));
-        return fail(DSIM_ERR_OVERFLOW, "two fibres of one membrane type enclose a spin");
-    }
-    s->moment_ready = false;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_sync(dsim_sim *s) {
-    if (int rc = check_sim(s)) return rc;
-    return sync_and_check(s);
-}
-
-extern "C" int dsim_step_timing(dsim_sim *s, double *ms, uint64_t *launches) {
-    if (int rc = check_sim(s)) return rc;
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    for (auto &e : s->ev) {
-        float t = 0.f;
-        HIP_TRY(hipEventElapsedTime(&t, e.first, e.second));
-        s->ms_done += t;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    s->ev.clear();
-    if (ms) *ms = s->ms_done;
-    if (launches) *launches = s->launches;
-    return DSIM_OK;
-}
-
-extern "C" int dsim_step_timing_reset(dsim_sim *s) {
-    if (int rc = check_sim(s)) return rc;
-    double t;
-    if (int rc = dsim_step_timing(s, &t, nullptr)) return rc;
-    s->ms_done = 0.0;
-    s->launches = 0;
-    return DSIM_OK;
-}

@@ -1,0 +1,79 @@
+// Shared pieces of the host runtime: error reporting, device buffers and the
+// substrate object (used by host.cpp and sim.cpp).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+
+#include "../../include/dsim.h"
+
+namespace dsim_host {
+
+// Records a message for dsim_last_error() and returns code.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        n = count;
+        if (count == 0) return DSIM_OK;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            return fail(DSIM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        return DSIM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+int upload(DevBuf<T> &d, const std::vector<T> &h) {
+    int rc = d.alloc(h.size());
+    if (rc) return rc;
+    if (!h.empty()) {
+        hipError_t e = hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(DSIM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    return DSIM_OK;
+}
+
+}  // namespace dsim_host
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return dsim_host::fail(DSIM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct dsim_substrate {
+    uint32_t n_tri = 0, n_vert = 0, n_fibers = 0;
+    std::vector<float> helpers;  // [n_tri*12]
+    std::vector<uint32_t> cell_start, cell_tri;
+    uint32_t n_cells = 0, max_per_cell = 0;
+    uint32_t kind = 0;
+    uint32_t dims[3] = {0, 0, 0};
+    float lo[3] = {0, 0, 0}, cell[3] = {0, 0, 0}, inv_cell[3] = {0, 0, 0};
+    dsim_host::DevBuf<float4> tri_ref, tri_mt, tri_aabb, cell_plane;
+    dsim_host::DevBuf<uint32_t> d_cell_start, d_cell_tri;
+    dsim_host::DevBuf<uint8_t> fiber_myelin;
+    int refs = 0;
+    bool on_device = false;
+    ~dsim_substrate() {
+        tri_ref.release(); tri_mt.release(); tri_aabb.release(); cell_plane.release();
+        d_cell_start.release(); d_cell_tri.release(); fiber_myelin.release();
+    }
+};

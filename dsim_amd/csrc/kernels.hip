@@ -6,6 +6,7 @@
 // path's hit triangles, cells, labels, positions and phases bit for bit; the
 // physical-mode code spells its fused multiply-adds out with fmaf().
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstdint>
 
@@ -157,14 +158,32 @@ __device__ __forceinline__ void test_ref(const float4 *__restrict__ R, uint32_t 
     }
 }
 
+// Plane-side prefilter: does the segment o-p cross the plane (n, n.v0)?
+__device__ __forceinline__ bool crosses_plane(const float4 pl, float ox, float oy, float oz,
+                                              float px, float py, float pz) {
+    float s0 = fmaf(pl.x, ox, fmaf(pl.y, oy, fmaf(pl.z, oz, -pl.w)));
+    float s1 = fmaf(pl.x, px, fmaf(pl.y, py, fmaf(pl.z, pz, -pl.w)));
+    return (s0 > 0.0f) != (s1 > 0.0f);
+}
+
+// Moller-Trumbore on the open segment 0<t<1, for a triangle whose plane the
+// segment crosses.
+__device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, uint32_t t,
+                                                 float ox, float oy, float oz, float px, float py,
+                                                 float pz, Hit &h);
+
 // Plane-side prefilter, then Moller-Trumbore on the open segment 0<t<1.
 __device__ __forceinline__ void test_mt(const float4 *__restrict__ M, uint32_t t, float ox,
                                         float oy, float oz, float px, float py, float pz, Hit &h) {
     h.type = 0;
-    const float4 pl = M[t * 4 + 0];
-    float s0 = fmaf(pl.x, ox, fmaf(pl.y, oy, fmaf(pl.z, oz, -pl.w)));
-    float s1 = fmaf(pl.x, px, fmaf(pl.y, py, fmaf(pl.z, pz, -pl.w)));
-    if ((s0 > 0.0f) == (s1 > 0.0f)) return;
+    if (!crosses_plane(M[t * 4 + 0], ox, oy, oz, px, py, pz)) return;
+    test_mt_crossing(M, t, ox, oy, oz, px, py, pz, h);
+}
+
+__device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, uint32_t t,
+                                                 float ox, float oy, float oz, float px, float py,
+                                                 float pz, Hit &h) {
+    h.type = 0;
     const float4 v0 = M[t * 4 + 1];
     const float4 e1 = M[t * 4 + 2];
     const float4 e2 = M[t * 4 + 3];
@@ -213,14 +232,22 @@ __device__ __forceinline__ void cell_test(const StepParams &P, uint32_t cell, ui
                                           float pz, Hit &best) {
     uint32_t k0 = P.cell_start[cell], k1 = P.cell_start[cell + 1];
     for (uint32_t k = k0; k < k1; ++k) {
-        uint32_t t = P.cell_tri[k];
-        if (t == excluded) continue;
-        Hit c;
-        if (MODE == 0)
+        if (MODE == 0) {
+            uint32_t t = P.cell_tri[k];
+            if (t == excluded) continue;
+            Hit c;
             test_ref(P.tri_ref, t, ox, oy, oz, px, py, pz, c);
-        else
-            test_mt(P.tri_mt, t, ox, oy, oz, px, py, pz, c);
-        if (better<MODE>(c, best)) best = c;
+            if (better<MODE>(c, best)) best = c;
+        } else {
+            // the cell-major plane record rejects most candidates without
+            // touching the triangle record
+            if (!crosses_plane(P.cell_plane[k], ox, oy, oz, px, py, pz)) continue;
+            uint32_t t = P.cell_tri[k];
+            if (t == excluded) continue;
+            Hit c;
+            test_mt_crossing(P.tri_mt, t, ox, oy, oz, px, py, pz, c);
+            if (better<MODE>(c, best)) best = c;
+        }
     }
 }
 
@@ -368,6 +395,8 @@ template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
+    // the spin's identity keys its random stream, injected normals and trace
+    const uint32_t sid = P.spin_id ? P.spin_id[i] : i;
     float x = P.x[i], y = P.y[i], z = P.z[i];
     float mag = P.mag[i];
     float ph = MOMENT ? 0.0f : P.phase[i];
@@ -386,12 +415,12 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
         float n0, n1, n2, u = 0.0f;
         bool have_u = true;
         if (RNG == kRngInject) {
-            float4 q = P.inject[(size_t)it * P.n + i];
+            float4 q = P.inject[(size_t)it * P.n + sid];
             n0 = q.x; n1 = q.y; n2 = q.z; u = q.w;
         } else if (RNG == kRngMwc) {
             mwc_randn(s0, s1, n0, n1, n2, u);
         } else {
-            philox_normals(P.key0, P.key1, i, gstep, n0, n1, n2);
+            philox_normals(P.key0, P.key1, sid, gstep, n0, n1, n2);
             have_u = false;
         }
         const float sd = comp == 0 ? P.std_dev[0] : comp == 1 ? P.std_dev[1]
@@ -404,7 +433,7 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
             px = fmaf(n0, sd, x); py = fmaf(n1, sd, y); pz = fmaf(n2, sd, z);
         }
         uint32_t first = kNone, last = kNone, ncoll = 0;
-        collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, i, gstep, comp, fiber, first,
+        collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber, first,
                            last, ncoll);
         if (!PERIODIC) {
             // spinKernelCpu.cpp:469-474
@@ -436,7 +465,7 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
         if (TRACE) {
             uint32_t qx, qy, qz;
             cell_of<MODE>(P, x, y, z, qx, qy, qz);
-            uint32_t *tr = P.trace + ((size_t)it * P.n + i) * 5;
+            uint32_t *tr = P.trace + ((size_t)it * P.n + sid) * 5;
             tr[0] = first; tr[1] = last; tr[2] = ncoll;
             tr[3] = cell_hash(P, qx, qy, qz);
             tr[4] = comp | (fiber << 8);
@@ -638,7 +667,10 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
                     rminz > hi.z || lo.z > rmaxz)
                     continue;
                 Hit h;
-                test_ref(P.tri_ref, t0 + k, sx, sy, sz, ex, ey, ez, h);
+                if (P.use_mt)
+                    test_mt(P.tri_mt, t0 + k, sx, sy, sz, ex, ey, ez, h);
+                else
+                    test_ref(P.tri_ref, t0 + k, sx, sy, sz, ex, ey, ez, h);
                 if (h.type == 0) continue;
                 uint32_t info = __float_as_uint(lo.w);
                 uint32_t f = info & 0xFFFFu, m = info >> 16;
@@ -650,17 +682,17 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
         }
         if (!done) {
             uint32_t comp = 0, fib = kFiberNone, nedge = 0;
-            bool found = false;
+            bool found = false, multi = false;
             for (uint32_t m = 0; m < 3 && !found; ++m) {
                 nedge += amb[m];
                 if (xf[m] != 0u) {
                     uint32_t f = xf[m] - 1u;
-                    if (fiber_hash(f) != xh[m]) atomicOr(P.flags, 2u);
+                    multi |= fiber_hash(f) != xh[m];
                     found = true;
                     fib = f;
                     comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
                 } else if (xh[m] != 0u) {
-                    atomicOr(P.flags, 2u);
+                    multi = true;
                 }
             }
             out_comp = comp;
@@ -671,6 +703,8 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
                 tries++;
                 if (tries > P.max_tries) done = true;
             }
+            // parity sums are only trusted on a ray that touched no edge
+            if (nedge == 0 && multi) atomicOr(P.flags, 2u);
         }
     }
     if (live) {
@@ -688,9 +722,11 @@ hipError_t launch_label(const LabelParams &p, hipStream_t s) {
 // resetSpins position draw (spinSystem.cpp:1386-1398) from Philox stream 2.
 __global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, float *y, float *z,
                                                       float *mag, float *phase, uint32_t *label,
-                                                      float box, uint32_t k0, uint32_t k1) {
+                                                      uint32_t *id, float box, uint32_t k0,
+                                                      uint32_t k1) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    id[i] = i;
     uint4 o = philox(make_uint4(i, 0u, 0u, 2u), k0, k1);
     float b2 = box * 2.0f;
     x[i] = b2 * (u32_unit(o.x) - 0.5f);
@@ -702,19 +738,20 @@ __global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, floa
 }
 
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
-                               uint32_t *label, float start_box, uint32_t key0, uint32_t key1,
-                               hipStream_t s) {
+                               uint32_t *label, uint32_t *id, float start_box, uint32_t key0,
+                               uint32_t key1, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_init_uniform, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, mag,
-                       phase, label, start_box, key0, key1);
+                       phase, label, id, start_box, key0, key1);
     return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_moment_reset(uint32_t n, const float *x, const float *y,
                                                       const float *z, float *x0, float *disp,
-                                                      float *mom, float *momc) {
+                                                      float *mom, float *momc, float *mag) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    mag[i] = 1.0f;
     x0[i] = x[i]; x0[n + i] = y[i]; x0[2 * n + i] = z[i];
     for (int k = 0; k < 3; ++k) {
         disp[k * n + i] = 0.0f;
@@ -724,10 +761,62 @@ __global__ __launch_bounds__(256) void k_moment_reset(uint32_t n, const float *x
 }
 
 hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
-                               float *x0, float *disp, float *mom, float *momc, hipStream_t s) {
+                               float *x0, float *disp, float *mom, float *momc, float *mag,
+                               hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_moment_reset, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, x0,
-                       disp, mom, momc);
+                       disp, mom, momc, mag);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------ //
+// Spatial reordering of spins (Morton order of their grid cells), so that the
+// lanes of a wavefront walk the same cells and triangle records.
+// ------------------------------------------------------------------------ //
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_sort_key(StepParams P, uint32_t *keys, uint32_t *vals) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t cx, cy, cz;
+    cell_of<1>(P, P.x[i], P.y[i], P.z[i], cx, cy, cz);
+    keys[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    vals[i] = i;
+}
+
+hipError_t sort_temp_bytes(uint32_t n, size_t *bytes) {
+    return rocprim::radix_sort_pairs(nullptr, *bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                     (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 30);
+}
+
+hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *keys_out,
+                            uint32_t *vals_in, uint32_t *vals_out, void *temp, size_t temp_bytes,
+                            hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_key, dim3((p.n + 255) / 256), dim3(256), 0, s, p, keys_in, vals_in);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, p.n,
+                                     0, 30, s);
+}
+
+__global__ __launch_bounds__(256) void k_gather(uint32_t n, const uint32_t *perm, Fields f) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t src = perm[j];
+    for (uint32_t q = 0; q < f.count; ++q) f.dst[q][j] = f.src[q][src];
+}
+
+hipError_t launch_gather(uint32_t n, const uint32_t *perm, const Fields &f, hipStream_t s) {
+    if (n == 0 || f.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, f);
     return hipGetLastError();
 }
 

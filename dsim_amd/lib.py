@@ -58,7 +58,8 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_float), ("std_dev", C.c_float * 4), ("t2", C.c_float * 4),
                 ("permeability", C.c_float), ("reflection_type", C.c_uint32), ("mode", C.c_uint32),
                 ("boundary", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint64),
-                ("spin_radius", C.c_float)]
+                ("spin_radius", C.c_float), ("sort_interval", C.c_uint32),
+                ("steps_per_launch", C.c_uint32)]
 
 
 _lib = None
@@ -69,9 +70,14 @@ _vp = C.c_void_p
 
 
 def exported_names():
-    """Function names declared in include/dsim.h."""
-    src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dsim_\w+)\s*\(", src, re.M)))
+    """Function names declared in include/*.h."""
+    inc = os.path.dirname(HEADER)
+    names = set()
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            src = open(os.path.join(inc, h)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dsim_\w+)\s*\(", src, re.M))
+    return sorted(names)
 
 
 def lib():
@@ -194,7 +200,7 @@ class Simulation:
 
     def __init__(self, substrate: Substrate, n, *, dt, std_dev, t2=(np.inf,) * 4, permeability=0.0,
                  reflection_type=1, mode=MODE_REFERENCE, boundary=BOUNDARY_CLAMP, rng=RNG_PHILOX,
-                 seed=1, spin_radius=0.005):
+                 seed=1, spin_radius=0.005, sort_interval=0, steps_per_launch=0):
         self.substrate = substrate
         self.n = int(n)
         p = Params()
@@ -208,6 +214,8 @@ class Simulation:
         p.rng = rng
         p.seed = int(seed) & ((1 << 64) - 1)
         p.spin_radius = float(spin_radius)
+        p.sort_interval = int(sort_interval)
+        p.steps_per_launch = int(steps_per_launch)
         self.params = p
         h = _vp()
         check(lib().dsim_sim_create(substrate.h, C.byref(p), self.n, C.byref(h)))
@@ -323,3 +331,154 @@ class Simulation:
 
     def step_timing_reset(self):
         check(lib().dsim_step_timing_reset(self.h))
+
+
+# --------------------------------------------------------------------------- #
+# I/O (include/dsim_io.h)
+# --------------------------------------------------------------------------- #
+PATH_MAX = 1024
+
+
+class Config(C.Structure):
+    _fields_ = [("stepsPerUpdate", C.c_int32), ("useGpu", C.c_int32), ("useDisplay", C.c_int32),
+                ("writeOut", C.c_int32), ("numSpins", C.c_uint32),
+                ("gyroMagneticRatio", C.c_float), ("timeStep", C.c_float),
+                ("extraAdc", C.c_float), ("intraAdc", C.c_float), ("myelinAdc", C.c_float),
+                ("extraT2", C.c_float), ("intraT2", C.c_float), ("myelinT2", C.c_float),
+                ("spaceScale", C.c_float), ("startBox", C.c_float), ("permeability", C.c_float),
+                ("searchMethod", C.c_uint32), ("outFile", C.c_char * PATH_MAX),
+                ("gradsFile", C.c_char * PATH_MAX), ("fiberFile", C.c_char * PATH_MAX),
+                ("configFile", C.c_char * PATH_MAX), ("seed", C.c_uint64)]
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32), ("n_axon", C.c_uint32),
+                ("n_myelin", C.c_uint32), ("n_glia", C.c_uint32), ("n_fibers", C.c_uint32),
+                ("max_triangles_on_surface", C.c_uint32), ("xmax", C.c_float),
+                ("ymax", C.c_float), ("zmax", C.c_float)]
+
+
+class TensorFit(C.Structure):
+    _fields_ = [("tensor", (C.c_double * 3) * 3), ("eigvec", (C.c_double * 3) * 3),
+                ("eigval", C.c_float * 3), ("md", C.c_float), ("fa", C.c_float),
+                ("rd", C.c_float)]
+
+
+_io_ready = False
+
+
+def _io():
+    global _io_ready
+    L = lib()
+    if not _io_ready:
+        L.dsim_io_last_error.restype = C.c_char_p
+        sigs = {
+            "dsim_config_defaults": [C.POINTER(Config)],
+            "dsim_config_read": [C.c_char_p, C.POINTER(Config)],
+            "dsim_config_apply_args": [C.POINTER(Config), C.c_int, C.POINTER(C.c_char_p)],
+            "dsim_grads_read": [C.c_char_p, _fp, C.c_uint32, _up],
+            "dsim_mesh_read": [C.c_char_p, C.c_float, C.POINTER(_vp)],
+            "dsim_mesh_view": [_vp, C.POINTER(Mesh), C.POINTER(MeshInfo)],
+            "dsim_mesh_free": [_vp],
+            "dsim_mesh_write": [C.c_char_p, C.POINTER(Mesh), C.c_float],
+            "dsim_pgse_schedule": [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                   C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)],
+            "dsim_tensor_fit_compute": [C.c_uint32, _fp, _fp, C.POINTER(TensorFit)],
+            "dsim_write_m": [C.c_char_p, C.POINTER(Config), C.c_uint32, _fp, _fp,
+                             C.POINTER(TensorFit)],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _io_ready = True
+    return L
+
+
+def io_check(rc):
+    if rc != OK:
+        raise DsimError(rc, _io().dsim_io_last_error().decode(errors="replace"))
+    return rc
+
+
+def read_config(path=None, argv=()):
+    c = Config()
+    io_check(_io().dsim_config_defaults(C.byref(c)))
+    if path is not None:
+        io_check(_io().dsim_config_read(str(path).encode(), C.byref(c)))
+    if argv:
+        arr = (C.c_char_p * (len(argv) + 1))(b"dsim", *[a.encode() for a in argv])
+        io_check(_io().dsim_config_apply_args(C.byref(c), len(argv) + 1, arr))
+    return c
+
+
+def read_grads(path, cap=1 << 16):
+    rows = np.zeros(cap * 6, np.float32)
+    n = C.c_uint32()
+    io_check(_io().dsim_grads_read(str(path).encode(), _p(rows, C.c_float), cap, C.byref(n)))
+    return rows[: n.value * 6].reshape(-1, 6)
+
+
+def read_mesh(path, space_scale):
+    h = _vp()
+    io_check(_io().dsim_mesh_read(str(path).encode(), float(space_scale), C.byref(h)))
+    try:
+        m, info = Mesh(), MeshInfo()
+        io_check(_io().dsim_mesh_view(h, C.byref(m), C.byref(info)))
+        nv, nt = info.n_vertices, info.n_triangles
+        out = {k: getattr(info, k) for k, _ in MeshInfo._fields_}
+        out["vertices"] = (np.ctypeslib.as_array(m.vertices, (nv * 3,)).reshape(-1, 3).copy()
+                           if nv else np.zeros((0, 3), np.float32))
+        out["triangles"] = (np.ctypeslib.as_array(m.triangles, (nt * 3,)).reshape(-1, 3).copy()
+                            if nt else np.zeros((0, 3), np.uint32))
+        out["tri_info"] = (np.ctypeslib.as_array(m.tri_info, (nt * 3,)).reshape(-1, 3).copy()
+                           if nt else np.zeros((0, 3), np.uint32))
+        return out
+    finally:
+        _io().dsim_mesh_free(h)
+
+
+def write_mesh(path, vertices, triangles, tri_info, space_scale):
+    V = np.ascontiguousarray(vertices, np.float32)
+    T = np.ascontiguousarray(triangles, np.uint32)
+    I = np.ascontiguousarray(tri_info, np.uint32)
+    m = Mesh(len(V), _p(V, C.c_float), len(T), _p(T, C.c_uint32), _p(I, C.c_uint32))
+    io_check(_io().dsim_mesh_write(str(path).encode(), C.byref(m), float(space_scale)))
+
+
+def pgse_schedule(ldelta, bdelta, readout, time_step, steps_per_update, cap=1 << 22):
+    sign = np.zeros(cap, np.int32)
+    L, B, R, ro = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    io_check(_io().dsim_pgse_schedule(ldelta, bdelta, readout, time_step, steps_per_update,
+                                      sign.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(L),
+                                      C.byref(B), C.byref(R), C.byref(ro)))
+    n = L.value + B.value + R.value
+    return sign[:n].copy(), (L.value, B.value, R.value, ro.value)
+
+
+def tensor_fit(rows, signal):
+    R = np.ascontiguousarray(rows, np.float32)
+    S_ = np.ascontiguousarray(signal, np.float32)
+    t = TensorFit()
+    io_check(_io().dsim_tensor_fit_compute(len(R), _p(R, C.c_float), _p(S_, C.c_float),
+                                           C.byref(t)))
+    return dict(tensor=np.array([list(r) for r in t.tensor]), eigvec=np.array([list(r) for r in t.eigvec]),
+                eigval=np.array(t.eigval[:]), md=t.md, fa=t.fa, rd=t.rd)
+
+
+def write_m(path, config, rows, signals, fits):
+    R = np.ascontiguousarray(rows, np.float32)
+    Sg = np.ascontiguousarray(signals, np.float32)
+    arr = (TensorFit * len(fits))()
+    for k, f in enumerate(fits):
+        for i in range(3):
+            for j in range(3):
+                arr[k].tensor[i][j] = f["tensor"][i][j]
+                arr[k].eigvec[i][j] = f["eigvec"][i][j]
+        for i in range(3):
+            arr[k].eigval[i] = f["eigval"][i]
+        arr[k].md, arr[k].fa, arr[k].rd = f["md"], f["fa"], f["rd"]
+    io_check(_io().dsim_write_m(str(path).encode(), C.byref(config), len(R), _p(R, C.c_float),
+                                _p(Sg, C.c_float), arr))

@@ -220,11 +220,16 @@ def protocol(cfg: Config):
     return np.array(rows, np.float32)
 
 
-def build(cfg: Config, halo_cells=2, cell_um=None):
-    """Mesh + grid parameters for a configuration."""
+def pack(cfg: Config) -> Axons:
+    """The configuration's axon packing (seeded, deterministic)."""
     rng = np.random.default_rng(cfg.seed)
-    ax = make_axons(cfg.n_axons, cfg.gamma_k, cfg.mean_radius, rng, voxel=cfg.voxel,
-                    fraction=cfg.fraction, ngon=cfg.ngon)
+    return make_axons(cfg.n_axons, cfg.gamma_k, cfg.mean_radius, rng, voxel=cfg.voxel,
+                      fraction=cfg.fraction, ngon=cfg.ngon)
+
+
+def build(cfg: Config, halo_cells=2, cell_um=None, axons: Axons | None = None):
+    """Mesh + grid parameters for a configuration (axons: reuse a packing)."""
+    ax = axons if axons is not None else pack(cfg)
     L = ax.voxel
     s = L / 2.0
     sigma_um = np.sqrt(2 * cfg.D * cfg.dt)

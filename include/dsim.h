@@ -94,6 +94,9 @@ typedef struct {
     uint32_t rng;             /* DSIM_RNG_*; MWC is the reference's per-spin stream */
     uint64_t seed;            /* Philox key */
     float spin_radius;        /* clamp margin (spinSystem.cpp:61: 0.005) */
+    uint32_t sort_interval;   /* physical mode with Philox: reorder spins by cell every
+                               * this many steps (0 = never).  Results do not depend on it. */
+    uint32_t steps_per_launch; /* steps fused per kernel launch (0 = 256) */
 } dsim_params;
 
 typedef struct dsim_substrate dsim_substrate;
@@ -173,7 +176,9 @@ int dsim_integrate_injected(dsim_sim *sim, const float grad_scaled[3], float pha
 int dsim_signal(dsim_sim *sim, double out[5]);
 
 /* ---- shared-walk gradient moments (one walk serves every direction) ------ */
-/* Zeroes the displacement and moment accumulators and records x0. */
+/* Starts an acquisition (the per-direction resetSpins of diffusion.cpp:565
+ * without moving the spins): records x0, zeroes the displacement and moment
+ * accumulators and sets every magnitude to 1.  Positions and labels stay. */
 int dsim_moment_reset(dsim_sim *sim);
 /* Advance `iterations` steps accumulating M += weight * (x - x0) per step,
  * weight in {+1, 0, -1} for the PGSE lobes of diffusion.cpp:564-585. */

@@ -77,7 +77,7 @@ class StateC(C.Structure):
 class StepArgsC(C.Structure):
     _fields_ = [("rng", C.c_uint32), ("philox_key", C.c_uint32 * 2), ("step0", C.c_uint64),
                 ("inject", _fp), ("grad", F3), ("phase_constant", C.c_float), ("weight", C.c_float),
-                ("trace", _up), ("n_threads", C.c_int)]
+                ("trace", _up), ("n_threads", C.c_int), ("stats", C.POINTER(C.c_uint64))]
 
 
 class MeshC(C.Structure):
@@ -411,7 +411,8 @@ class State:
 
 
 def integrate(world: World, state: State, iters, *, rng=RNG_INJECT, inject=None, key=(0, 0),
-              step0=0, grad=(0, 0, 0), phase_constant=0.0, weight=0.0, trace=False, n_threads=0):
+              step0=0, grad=(0, 0, 0), phase_constant=0.0, weight=0.0, trace=False, n_threads=0,
+              stats=None):
     a = StepArgsC()
     a.rng = rng
     a.philox_key = (C.c_uint32 * 2)(*[int(k) for k in key])
@@ -429,6 +430,9 @@ def integrate(world: World, state: State, iters, *, rng=RNG_INJECT, inject=None,
         tr = np.zeros((iters, state.n, 5), np.uint32)
         a.trace = _p(tr, C.c_uint32)
     a.n_threads = n_threads
+    if stats is not None:
+        assert stats.dtype == np.uint64 and stats.size >= 3
+        a.stats = stats.ctypes.data_as(C.POINTER(C.c_uint64))
     sc = state.c()
     lib().orc_integrate(C.byref(world.c), C.byref(sc), iters, C.byref(a))
     return tr

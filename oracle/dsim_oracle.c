@@ -617,6 +617,45 @@ orc_f3 orc_coll_detect_brute(const orc_world *w, orc_f3 o, orc_f3 p, float u, ui
 /* Stepping                                                                  */
 /* ======================================================================== */
 
+/* Number of distinct triangles whose bounding box overlaps the box of the
+ * segment o-p: the candidate count of an exact bounding-box search (what the
+ * reference's R-tree query returns, spinKernelCpu.cpp:125), independent of the
+ * grid.  Work-model counter only. */
+static uint32_t aabb_candidates(const orc_world *w, orc_f3 o, orc_f3 p)
+{
+    float mn[3] = {o.x, o.y, o.z}, mx[3] = {o.x, o.y, o.z};
+    if (p.x < mn[0]) mn[0] = p.x; if (p.y < mn[1]) mn[1] = p.y; if (p.z < mn[2]) mn[2] = p.z;
+    if (p.x > mx[0]) mx[0] = p.x; if (p.y > mx[1]) mx[1] = p.y; if (p.z > mx[2]) mx[2] = p.z;
+    uint32_t c0[3], c1[3];
+    orc_cube_pos(w, f3(mn[0], mn[1], mn[2]), c0);
+    orc_cube_pos(w, f3(mx[0], mx[1], mx[2]), c1);
+    uint32_t buf[4096];
+    uint32_t nb = 0;
+    for (uint32_t z = c0[2]; z <= c1[2]; ++z)
+        for (uint32_t y = c0[1]; y <= c1[1]; ++y)
+            for (uint32_t x = c0[0]; x <= c1[0]; ++x) {
+                uint32_t cc[3] = {x, y, z}, h = orc_cube_hash(w, cc);
+                for (uint32_t k = 0; k < w->cell_count[h] && nb < 4096; ++k) {
+                    uint32_t t = w->cell_list[(size_t)h * w->max_per_cell + k], dup = 0;
+                    for (uint32_t q = 0; q < nb; ++q)
+                        if (buf[q] == t) { dup = 1; break; }
+                    if (!dup) buf[nb++] = t;
+                }
+            }
+    uint32_t cnt = 0;
+    for (uint32_t q = 0; q < nb; ++q) {
+        const uint32_t *T = w->trgls + (size_t)buf[q] * 3;
+        int ok = 1;
+        for (int k = 0; k < 3 && ok; ++k) {
+            float a = w->vertices[T[0] * 3 + k], b = w->vertices[T[1] * 3 + k], c = w->vertices[T[2] * 3 + k];
+            float lo = fminf(a, fminf(b, c)), hi = fmaxf(a, fmaxf(b, c));
+            if (lo > mx[k] || mn[k] > hi) ok = 0;
+        }
+        cnt += (uint32_t)ok;
+    }
+    return cnt;
+}
+
 /* cpuIntegrate, dSim3/src/spinKernelCpu.cpp:430-499, one spin one step. */
 static void step_one(const orc_world *w, orc_state *s, uint32_t i, uint32_t it, const orc_step_args *a)
 {
@@ -646,7 +685,24 @@ static void step_one(const orc_world *w, orc_state *s, uint32_t i, uint32_t it, 
          * would change nothing, so draw it up front for simplicity */
         u = orc_philox_step_uniform(a->philox_key, i, gstep);
     }
+    if (a->stats) {
+        uint32_t cand = aabb_candidates(w, o, pos);
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        a->stats[1] += cand;
+    }
     pos = orc_coll_detect_grid(w, o, pos, u, &comp, &fiber, &hits);
+    if (a->stats) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        a->stats[0] += 1;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        a->stats[2] += hits.ncoll;
+    }
     if (w->boundary == 0) {
         /* spinKernelCpu.cpp:469-474 */
         float hi = 1.0f - w->spin_radius, lo = -1.0f + w->spin_radius;
@@ -796,7 +852,9 @@ uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float 
                     }
                     uint32_t ncoll = 0;
                     for (uint32_t p = 0; p < ng; ++p) {
-                        orc_coll r = orc_tri_coll_detect(w, sp, le, grp[p]);
+                        /* the physical mode (test 1) counts crossings with its own
+                         * Moller-Trumbore test */
+                        orc_coll r = test_tri(w, sp, le, grp[p]);
                         if (r.type > 0) {
                             ncoll++;
                             if (r.type == 2) nedge++;

@@ -158,6 +158,8 @@ typedef struct {
     float weight;               /* moment mode lobe weight (+1, 0, -1) */
     uint32_t *trace;            /* [iters][n][5] first,last,ncoll,cell,label or NULL */
     int n_threads;              /* OpenMP threads (<=0: default) */
+    uint64_t *stats;            /* NULL or [3]: steps, triangles whose box overlaps the
+                                 * step segment's box, membrane hits (work model counters) */
 } orc_step_args;
 
 void orc_integrate(const orc_world *w, orc_state *s, uint32_t iters,

@@ -35,7 +35,7 @@ def _mesh(ref_inputs, name, scale):
 
 def _pair(m, *, mode=0, grid="reference", dims=None, lo=None, cell=None, eps=1e-5,
           std=(0.03, 0.03, 0.01, 0.03), t2=(80.0, 70.0, 7.5, 70.0), perm=0.2, boundary=0,
-          rng=D.RNG_PHILOX, n=0, seed=1):
+          rng=D.RNG_PHILOX, n=0, seed=1, sort_interval=0):
     kw = dict(std_dev=std, t2=t2, dt=DT, permeability=perm)
     if grid == "reference":
         w = O.World(m["vertices"], m["trgls"], m["tri_info"], **kw)
@@ -48,7 +48,7 @@ def _pair(m, *, mode=0, grid="reference", dims=None, lo=None, cell=None, eps=1e-
     if mode == 1:
         w.set(test=1, traversal=1, transitions=1, boundary=boundary)
     sim = D.Simulation(sub, n, dt=DT, std_dev=std, t2=t2, permeability=perm, mode=mode,
-                       boundary=boundary, rng=rng, seed=seed)
+                       boundary=boundary, rng=rng, seed=seed, sort_interval=sort_interval)
     return w, sub, sim
 
 
@@ -94,14 +94,16 @@ def test_injected_reference_mode_bit_exact(ref_inputs, name, scale):
     np.testing.assert_array_equal(g["fiber"], st.fiber)
 
 
+@pytest.mark.parametrize("sort", [0, 16])
 @pytest.mark.parametrize("boundary", [0, 1])
 @pytest.mark.parametrize("name,scale", CASES[:2] + [("sim.triangles_curved", 1.0)])
-def test_injected_physical_mode_bit_exact(ref_inputs, name, scale, boundary):
-    _say(f"case physical-injected {name} scale={scale} boundary={boundary}")
+def test_injected_physical_mode_bit_exact(ref_inputs, name, scale, boundary, sort):
+    _say(f"case physical-injected {name} scale={scale} boundary={boundary} sort={sort}")
     m = _mesh(ref_inputs, name, scale)
     n, iters = 3000, 48
     dims, lo, cell = (37, 29, 13), (-1.2, -1.2, -1.2), (2.4 / 37, 2.4 / 29, 2.4 / 13)
-    w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=boundary, n=n)
+    w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=boundary, n=n,
+                        sort_interval=sort)
     pos, comp, fib = _spins(w, m, n, 9, plus_x=True)
     q = _inject(n, iters, 10)
     st = O.State(pos, comp=comp, fiber=fib, moment=True)
@@ -242,3 +244,31 @@ def test_free_diffusion_signal_matches_exp_minus_bD():
     want = np.exp(-b * D_adc)
     sigma = np.sqrt((1 - want ** 2) / (2 * n)) + 1e-4
     assert np.all(np.abs(S - want) < 3 * sigma + 2e-3), (S, want)
+
+
+def test_sorted_philox_run_is_order_independent(ref_inputs):
+    """Spatial reordering changes only where spins live in memory: per-spin
+    results (keyed by spin id) are bit-identical with and without it."""
+    _say("case sort-invariance")
+    m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
+    n = 20000
+    dims, lo, cell = (41, 41, 9), (-1.2, -1.2, -1.2), (2.4 / 41, 2.4 / 41, 2.4 / 9)
+    outs = []
+    for sort in (0, 32):
+        w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=1, n=n,
+                            sort_interval=sort, seed=77)
+        sim.init_uniform(1.0)
+        sim.label(m["xmax"], m["ymax"], m["zmax"])
+        sim.moment_reset()
+        sim.integrate_moment(1.0, 100)
+        sim.integrate_moment(-1.0, 100)
+        st = sim.get_state()
+        mom, x0 = sim.get_moment()
+        sig = sim.signal_moment(np.array([[0.02, 0.01, 0.0]], np.float32), PC)
+        outs.append((st, mom, x0, sig))
+    (a, ma, xa, sa), (b, mb, xb, sb) = outs
+    for k in ("pos", "mag", "comp", "fiber"):
+        np.testing.assert_array_equal(a[k], b[k])
+    np.testing.assert_array_equal(ma, mb)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_allclose(sa, sb, rtol=1e-12)
