@@ -24,6 +24,7 @@ struct StepParams {
     const uint32_t *cell_start;
     const uint32_t *cell_tri;
     const float4 *cell_plane;  // plane (n, n.v0) of cell_tri[k], cell-major
+    const float *cell_clear;   // per-cell distance lower bound, or null (no skipping)
     const uint8_t *fiber_myelin;
     uint32_t n_fibers;
     uint32_t dims[3];

@@ -186,6 +186,64 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
     for (const Pair &p : pairs) tri[fill[p.cell]++] = p.tri;
 }
 
+// Per-cell clearance: a lower bound of the distance from any point of the cell
+// to any triangle, found as the smallest box-to-box distance between the cell
+// and the bounding boxes of the triangles listed in the cells around it
+// (rings of growing Chebyshev radius, stopping once a ring is farther than the
+// best found).  Geometry beyond the grid is unknown, so the distance to the
+// grid's outer face caps the value.  Rounded down to float.
+void compute_clearance(const uint32_t dims[3], const float lo[3], const float cell[3],
+                       const std::vector<uint32_t> &start, const std::vector<uint32_t> &tri,
+                       const std::vector<float4> &box, std::vector<float> &out) {
+    const uint64_t nc = (uint64_t)dims[0] * dims[1] * dims[2];
+    out.assign(nc, 0.0f);
+    const double hmin = std::min({(double)cell[0], (double)cell[1], (double)cell[2]});
+    const int64_t D[3] = {dims[0], dims[1], dims[2]};
+    unsigned nw = std::max(1u, std::min(n_workers(), (unsigned)((nc + 4095) / 4096)));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nw; ++w) {
+        th.emplace_back([&, w] {
+            for (uint64_t c = nc * w / nw; c < nc * (w + 1) / nw; ++c) {
+                int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
+                int64_t idx[3] = {ix, iy, iz};
+                double b0[3], b1[3];
+                double best = 1e30;
+                for (int k = 0; k < 3; ++k) {
+                    b0[k] = (double)lo[k] + (double)idx[k] * cell[k];
+                    b1[k] = b0[k] + cell[k];
+                    double glo = (double)lo[k], ghi = (double)lo[k] + (double)D[k] * cell[k];
+                    best = std::min(best, std::min(b0[k] - glo, ghi - b1[k]));
+                }
+                best = std::max(best, 0.0);
+                for (int64_t r = 0;; ++r) {
+                    if (r > 0 && (double)(r - 1) * hmin >= best) break;
+                    if (r > D[0] && r > D[1] && r > D[2]) break;
+                    for (int64_t z = std::max<int64_t>(0, iz - r); z <= std::min(D[2] - 1, iz + r); ++z)
+                        for (int64_t y = std::max<int64_t>(0, iy - r); y <= std::min(D[1] - 1, iy + r); ++y)
+                            for (int64_t x = std::max<int64_t>(0, ix - r); x <= std::min(D[0] - 1, ix + r); ++x) {
+                                if (std::max({std::llabs(x - ix), std::llabs(y - iy), std::llabs(z - iz)}) != r)
+                                    continue;  // only the ring at Chebyshev radius r
+                                uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
+                                for (uint32_t k = start[cc]; k < start[cc + 1]; ++k) {
+                                    const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
+                                    double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z}, d2 = 0;
+                                    for (int q = 0; q < 3; ++q) {
+                                        double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
+                                        d2 += g * g;
+                                    }
+                                    best = std::min(best, std::sqrt(d2));
+                                }
+                            }
+                }
+                float f = (float)best;
+                if ((double)f > best) f = std::nextafter(f, 0.0f);
+                out[c] = f;
+            }
+        });
+    }
+    for (auto &t : th) t.join();
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -352,6 +410,8 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     }
     S->n_cells = S->dims[0] * S->dims[1] * S->dims[2];
     pairs_to_csr(pairs, S->n_cells, S->cell_start, S->cell_tri, S->max_per_cell);
+    if (grid->kind == DSIM_GRID_AABB)
+        compute_clearance(S->dims, S->lo, S->cell, S->cell_start, S->cell_tri, box, S->clearance);
 
     if (!have_device) {  // host-only substrate: grid and helpers are inspectable, no sims
         *out = S;
@@ -360,7 +420,7 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     S->on_device = true;
     if ((rc = upload(S->tri_ref, ref)) || (rc = upload(S->tri_mt, mt)) ||
         (rc = upload(S->tri_aabb, box)) || (rc = upload(S->d_cell_start, S->cell_start)) ||
-        (rc = upload(S->fiber_myelin, myel))) {
+        (rc = upload(S->fiber_myelin, myel)) || (rc = upload(S->d_clearance, S->clearance))) {
         delete S;
         return rc;
     }
@@ -399,6 +459,13 @@ extern "C" int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_s
     if (!s) return fail(DSIM_ERR_ARG, "NULL substrate");
     if (cell_start) memcpy(cell_start, s->cell_start.data(), s->cell_start.size() * 4);
     if (cell_tri && !s->cell_tri.empty()) memcpy(cell_tri, s->cell_tri.data(), s->cell_tri.size() * 4);
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance) {
+    if (!s || !clearance) return fail(DSIM_ERR_ARG, "NULL argument");
+    if (s->clearance.empty()) return fail(DSIM_ERR_STATE, "the grid has no clearance table");
+    memcpy(clearance, s->clearance.data(), s->clearance.size() * 4);
     return DSIM_OK;
 }
 

@@ -387,6 +387,11 @@ __device__ __forceinline__ void collide(const StepParams &P, float ox, float oy,
     px = ex; py = ey; pz = ez;
 }
 
+// Margin kept below the cell clearance: far larger than the rounding of a
+// step (coordinates are O(1), ulp 1.2e-7), so a skipped step is one the full
+// search would also have found free.
+constexpr float kClearMargin = 1e-5f;
+
 // ------------------------------------------------------------------------ //
 // Step kernel: cpuIntegrate (spinKernelCpu.cpp:430-499) for `iters` steps
 // per launch with the spin's state held in registers.
@@ -409,6 +414,13 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
         dx = P.disp[i]; dy = P.disp[P.n + i]; dz = P.disp[2 * P.n + i];
         mx = P.mom[i]; my = P.mom[P.n + i]; mz = P.mom[2 * P.n + i];
         cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
+    }
+    // distance the spin may still travel without any triangle in reach
+    float safe = -1.0f;
+    if (MODE == 1 && P.cell_clear) {
+        uint32_t qx, qy, qz;
+        cell_of<1>(P, x, y, z, qx, qy, qz);
+        safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
     }
     for (uint32_t it = 0; it < iters; ++it) {
         const uint64_t gstep = P.step0 + it;
@@ -433,8 +445,22 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
             px = fmaf(n0, sd, x); py = fmaf(n1, sd, y); pz = fmaf(n2, sd, z);
         }
         uint32_t first = kNone, last = kNone, ncoll = 0;
-        collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber, first,
-                           last, ncoll);
+        bool refresh = false;
+        if (MODE == 1 && P.cell_clear) {
+            // No triangle lies within `safe` of the path walked since the last
+            // check, so a shorter step cannot hit one: skip the search.
+            const float len = sd * __builtin_amdgcn_sqrtf(fmaf(n0, n0, fmaf(n1, n1, n2 * n2)));
+            if (len < safe) {
+                safe -= len;
+            } else {
+                collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber,
+                                   first, last, ncoll);
+                refresh = true;
+            }
+        } else {
+            collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber,
+                               first, last, ncoll);
+        }
         if (!PERIODIC) {
             // spinKernelCpu.cpp:469-474
             if (px > P.clamp_hi) px = P.clamp_hi; else if (px < P.clamp_lo) px = P.clamp_lo;
@@ -445,11 +471,18 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
             dx += px - ox; dy += py - oy; dz += pz - oz;
         }
         if (PERIODIC) {
+            const float wx = px, wy = py, wz = pz;
             if (px >= 1.0f) px -= 2.0f; else if (px < -1.0f) px += 2.0f;
             if (py >= 1.0f) py -= 2.0f; else if (py < -1.0f) py += 2.0f;
             if (pz >= 1.0f) pz -= 2.0f; else if (pz < -1.0f) pz += 2.0f;
+            refresh |= (px != wx) | (py != wy) | (pz != wz);
         }
         x = px; y = py; z = pz;
+        if (MODE == 1 && P.cell_clear && refresh) {
+            uint32_t qx, qy, qz;
+            cell_of<1>(P, x, y, z, qx, qy, qz);
+            safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
+        }
         const float t2 = comp == 0 ? P.t2[0] : comp == 1 ? P.t2[1] : comp == 2 ? P.t2[2] : P.t2[3];
         const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
                        : comp == 2 ? P.decay[2] : P.decay[3];
@@ -697,14 +730,15 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
             }
             out_comp = comp;
             out_fiber = fib;
-            if (nedge == 0) {
+            // a ray touching an edge, or finding two odd fibres of one
+            // membrane type (a crossing counted twice or missed at a shared
+            // edge), is re-cast with a jitter
+            if (nedge == 0 && !multi) {
                 done = true;
             } else {
                 tries++;
                 if (tries > P.max_tries) done = true;
             }
-            // parity sums are only trusted on a ray that touched no edge
-            if (nedge == 0 && multi) atomicOr(P.flags, 2u);
         }
     }
     if (live) {

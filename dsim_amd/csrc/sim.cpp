@@ -290,6 +290,7 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.cell_start = b->d_cell_start.p;
     P.cell_tri = b->d_cell_tri.p;
     P.cell_plane = b->cell_plane.p;
+    P.cell_clear = s->prm.mode == DSIM_MODE_PHYSICAL ? b->d_clearance.p : nullptr;
     P.fiber_myelin = b->fiber_myelin.p;
     P.n_fibers = b->n_fibers;
     for (int k = 0; k < 3; ++k) {

@@ -189,6 +189,16 @@ class Substrate:
         check(lib().dsim_substrate_get_helpers(self.h, _p(h, C.c_float)))
         return h
 
+    def clearance(self):
+        nc, ne, mx = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().dsim_substrate_grid_info(self.h, C.byref(nc), C.byref(ne), C.byref(mx)))
+        out = np.zeros(nc.value, np.float32)
+        L = lib()
+        L.dsim_substrate_get_clearance.argtypes = [_vp, _fp]
+        L.dsim_substrate_get_clearance.restype = C.c_int
+        check(L.dsim_substrate_get_clearance(self.h, _p(out, C.c_float)))
+        return out
+
     def nbytes(self):
         b = C.c_uint64()
         check(lib().dsim_substrate_bytes(self.h, C.byref(b)))

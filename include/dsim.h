@@ -126,6 +126,11 @@ int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cells, uint32_
 int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_start, uint32_t *cell_tri);
 /* Copies m_triangleHelpers (12 floats per triangle, spinSystem.cpp:612-648). */
 int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers);
+/* AABB grids only (no reference counterpart): per-cell lower bound of the
+ * distance from any point of the cell to any triangle [n_cells].  The physical
+ * step skips collision detection while a spin's path since its last check is
+ * shorter than this clearance. */
+int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance);
 /* Device bytes held by the substrate. */
 int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes);
 
@@ -152,10 +157,10 @@ int dsim_init_spins_uniform(dsim_sim *sim, float start_box);
 /* findSpinsInFibers (spinSystem.cpp:1615-1852), on the device.  The reference
  * mode casts its ray along the spin's dominant axis to 1.1*max (as the
  * reference does); the physical mode casts it along +x so that open prisms
- * are counted.  Ambiguous rays (edge or vertex hits) are re-cast with a
- * reproducible jitter up to max_tries times; spins still ambiguous keep their
- * last label and are counted in *n_ambiguous.  Returns DSIM_ERR_OVERFLOW if a
- * ray found an odd crossing count on two fibres of one membrane type. */
+ * are counted.  Ambiguous rays (edge or vertex hits, or an odd crossing count
+ * on two fibres of one membrane type) are re-cast with a reproducible jitter
+ * up to max_tries times; spins still ambiguous keep their last label and are
+ * counted in *n_ambiguous. */
 int dsim_label_spins(dsim_sim *sim, float xmax, float ymax, float zmax, uint32_t max_tries,
                      uint32_t *n_ambiguous);
 

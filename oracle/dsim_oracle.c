@@ -772,7 +772,8 @@ static const uint32_t k_label_key[2] = {0x6A09E667u, 0xBB67AE85u};
  * triangles whose bounding box overlaps the ray's (the R-tree query of
  * spinSystem.cpp:1707, overlap test RTree.h:1550-1563), taken in index order.
  * The retry jitter replaces frand() by a Philox draw so that it is
- * reproducible, and a spin that stays ambiguous after max_tries keeps the
+ * reproducible, a ray that finds two odd fibres of one membrane type is
+ * re-cast as well, and a spin that stays ambiguous after max_tries keeps the
  * last attempt's label (the reference instead moves it to a new random
  * position, spinSystem.cpp:1628-1633). */
 uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float xmax, float ymax,
@@ -831,9 +832,10 @@ uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float 
                     continue;
                 hit[nh++] = t;
             }
-            uint32_t nedge = 0, nvert = 0, inside = ORC_FIBER_NONE;
+            uint32_t nedge = 0, nvert = 0, inside = ORC_FIBER_NONE, multi = 0;
             uint8_t sc = 0;
             for (uint32_t mt = 0; mt < 3 && inside == ORC_FIBER_NONE; ++mt) {
+                uint32_t nodd = 0;
                 for (uint32_t k = 0; k < nh; ++k) {
                     uint32_t ti = hit[k];
                     if (ti == ORC_NONE) continue;
@@ -864,13 +866,17 @@ uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float 
                     if (ncoll % 2 == 1) {
                         inside = fi;
                         sc = (mt == 0) ? 1 : (mt == 2) ? 3 : 2;
+                        nodd++;
                     }
                 }
+                /* two odd fibres of one membrane type: a crossing was counted
+                 * twice or missed at a shared edge; re-cast like an edge hit */
+                if (nodd > 1) multi = 1;
             }
             if (inside == ORC_FIBER_NONE) sc = 0;
             comp = sc;
             fib = (uint16_t)inside;
-            if (nedge == 0 && nvert == 0) break;
+            if (nedge == 0 && nvert == 0 && !multi) break;
             tries++;
             if (tries > max_tries) { ambiguous++; break; }
         }

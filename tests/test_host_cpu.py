@@ -66,3 +66,52 @@ def test_aabb_grid_matches_oracle(ref_inputs, name, scale):
     np.testing.assert_array_equal(np.diff(start), cnt)
     for c in np.nonzero(cnt)[0]:
         np.testing.assert_array_equal(tri[start[c]:start[c + 1]], lst[c, :cnt[c]])
+
+
+def _point_triangle_distance(p, a, b, c):
+    """Exact Euclidean distance from points p[n,3] to triangle abc (float64)."""
+    p = p.astype(np.float64)
+    a, b, c = (np.asarray(v, np.float64) for v in (a, b, c))
+    ab, ac = b - a, c - a
+    n = np.cross(ab, ac)
+    nn = n @ n
+    # projection onto the plane and its barycentrics
+    t = ((p - a) @ n) / nn
+    q = p - t[:, None] * n
+    v0, v1, v2 = ab, ac, q - a
+    d00, d01, d11 = v0 @ v0, v0 @ v1, v1 @ v1
+    d20, d21 = v2 @ v0, v2 @ v1
+    den = d00 * d11 - d01 * d01
+    v = (d11 * d20 - d01 * d21) / den
+    w = (d00 * d21 - d01 * d20) / den
+    inside = (v >= 0) & (w >= 0) & (v + w <= 1)
+    best = np.where(inside, np.abs(t) * np.sqrt(nn), np.inf)
+    for s0, s1 in ((a, b), (b, c), (c, a)):
+        e = s1 - s0
+        u = np.clip(((p - s0) @ e) / (e @ e), 0, 1)
+        best = np.minimum(best, np.linalg.norm(p - (s0 + u[:, None] * e), axis=1))
+    return best
+
+
+@pytest.mark.parametrize("name,scale", [("sim.fibers_1_r004_f09", 3.0), ("sim.triangles_curved", 1.0)])
+def test_clearance_is_a_distance_lower_bound(ref_inputs, name, scale):
+    m = O.parse_mesh(os.path.join(ref_inputs, name), scale)
+    dims, lo, cell = (31, 27, 9), (-1.2, -1.2, -1.2), (2.4 / 31, 2.4 / 27, 2.4 / 9)
+    sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims, lo=lo,
+                      cell=cell, eps=1e-5)
+    clr = sub.clearance()
+    assert clr.shape == (int(np.prod(dims)),) and (clr >= 0).all()
+    rng = np.random.default_rng(1)
+    V, T = m["vertices"], m["trgls"]
+    cells = rng.choice(len(clr), 400, replace=False)
+    lo_, ce = np.array(lo, np.float64), np.array(cell, np.float64)
+    tight = 0
+    for c in cells:
+        ix, iy, iz = c % dims[0], (c // dims[0]) % dims[1], c // (dims[0] * dims[1])
+        pts = lo_ + (np.array([ix, iy, iz]) + rng.uniform(0, 1, (64, 3))) * ce
+        d = np.full(len(pts), np.inf)
+        for t in T:
+            d = np.minimum(d, _point_triangle_distance(pts, *V[t]))
+        assert (clr[c] <= d + 1e-12).all(), (c, clr[c], d.min())
+        tight += clr[c] > 0.5 * d.min()
+    assert tight > 40  # the bound is useful, not just zero
