@@ -516,16 +516,342 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
     }
 }
 
+// ------------------------------------------------------------------------ //
+// Physical-mode walk: the same per-spin arithmetic as k_step<1,...>, laid
+// out for 64-wide wavefronts.
+//
+// Each lane runs a small state machine: it either starts its next step (draw
+// normals, propose the end point, take the clearance shortcut if it applies)
+// or has a segment of the current step pending a collision search.  Every
+// trip round the wave loop does one search round for all pending lanes
+// together, so a lane whose step ended without a hit starts its next step
+// while another lane is still reflecting.
+//
+// The search is compacted over the wave: each pending lane lists the CSR
+// ranges of the cells its segment's bounding box covers, a wave prefix sum
+// numbers the (lane, candidate) pairs, and the wave tests 64 pairs per pass,
+// whatever lane they belong to.  The nearest hit per lane is an LDS atomic
+// minimum over (t bits, triangle index): for t > 0 the float bits order like
+// the values, so this is the same (t, then index) order as the per-lane scan.
+// ------------------------------------------------------------------------ //
+constexpr uint32_t kWaveSegs = 256;  // cell ranges listed per wave and search round
+
+struct WaveScratch {
+    uint2 seg[kWaveSegs];  // (first entry - lane-local offset, lane-local end offset)
+    float4 o[64];          // segment start; .w = excluded triangle
+    float4 p[64];          // segment end
+    unsigned long long best[64];
+};
+
+// Orders this wave's LDS accesses: a wave's LDS instructions complete in
+// issue order, so only the compiler has to be kept from moving them.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exclusive prefix sum over the wave; every lane must be active.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane, uint32_t &total) {
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
+    return ((unsigned long long)__float_as_uint(t) << 32) | tri;
+}
+
+// One search round: nearest hit of each searching lane's segment o-p,
+// excluding `excluded`, as hit_key(t, triangle), or ~0 when there is none.
+// Called by all 64 lanes of the wave together.
+__device__ __forceinline__ unsigned long long wave_search(const StepParams &P, WaveScratch &L,
+                                                          uint32_t lane, bool search, float ox,
+                                                          float oy, float oz, float px, float py,
+                                                          float pz, uint32_t excluded) {
+    uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0, ncell = 0;
+    if (search) {
+        uint32_t sx, sy, sz, tx, ty, tz;
+        cell_of<1>(P, ox, oy, oz, sx, sy, sz);
+        cell_of<1>(P, px, py, pz, tx, ty, tz);
+        x0 = min(sx, tx); x1 = max(sx, tx);
+        y0 = min(sy, ty); y1 = max(sy, ty);
+        z0 = min(sz, tz); z1 = max(sz, tz);
+        ncell = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+    }
+    uint32_t nseg_total;
+    const uint32_t soff = wave_excl_scan(min(ncell, kWaveSegs + 1), lane, nseg_total);
+    // lanes whose cells do not fit the wave's list search on their own below
+    const bool coop = search && soff + ncell <= kWaveSegs;
+    uint32_t cnt = 0;
+    if (coop) {
+        uint32_t cx = x0, cy = y0, cz = z0;
+        for (uint32_t s = 0; s < ncell; s += 4) {
+            uint32_t k0[4], k1[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                k0[q] = k1[q] = 0;
+                if (s + q < ncell) {
+                    const uint32_t c = cell_hash(P, cx, cy, cz);
+                    k0[q] = P.cell_start[c];
+                    k1[q] = P.cell_start[c + 1];
+                    if (++cx > x1) { cx = x0; if (++cy > y1) { cy = y0; ++cz; } }
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                if (s + q < ncell) {
+                    L.seg[soff + s + q] = make_uint2(k0[q] - cnt, cnt + (k1[q] - k0[q]));
+                    cnt += k1[q] - k0[q];
+                }
+            }
+        }
+        L.o[lane] = make_float4(ox, oy, oz, __uint_as_float(excluded));
+        L.p[lane] = make_float4(px, py, pz, 0.0f);
+        L.best[lane] = ~0ull;
+    }
+    uint32_t total;
+    const uint32_t off = wave_excl_scan(cnt, lane, total);
+    wave_lds_sync();
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t j = base + lane;
+        // owner of pair j: the last lane whose range starts at or before j
+        uint32_t owner = 0, obase = 0;
+#pragma unroll
+        for (uint32_t st = 32; st; st >>= 1) {
+            const uint32_t c = owner + st;
+            const uint32_t v = __shfl(off, (int)c, 64);
+            if (v <= j) { owner = c; obase = v; }
+        }
+        const uint32_t oseg = __shfl(soff, (int)owner, 64);
+        if (j < total) {
+            const uint32_t r = j - obase;
+            uint32_t s = oseg;
+            uint2 e = L.seg[s];
+            while (r >= e.y) e = L.seg[++s];
+            const uint32_t k = e.x + r;
+            const float4 so = L.o[owner], sp = L.p[owner];
+            if (crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z)) {
+                const uint32_t t = P.cell_tri[k];
+                if (t != __float_as_uint(so.w)) {
+                    Hit h;
+                    test_mt_crossing(P.tri_mt, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
+                    if (h.type) atomicMin(&L.best[owner], hit_key(h.d, t));
+                }
+            }
+        }
+    }
+    wave_lds_sync();
+    unsigned long long key = ~0ull;
+    if (coop) {
+        key = L.best[lane];
+    } else if (search) {
+        Hit best;
+        best.type = 0;
+        best.idx = kNone;
+        best.d = 2.0f;
+        for (uint32_t cz = z0; cz <= z1; ++cz)
+            for (uint32_t cy = y0; cy <= y1; ++cy)
+                for (uint32_t cx = x0; cx <= x1; ++cx)
+                    cell_test<1>(P, cell_hash(P, cx, cy, cz), excluded, ox, oy, oz, px, py, pz,
+                                 best);
+        if (best.type) key = hit_key(best.d, best.idx);
+    }
+    return key;
+}
+
+// cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode, `iters` steps
+// per spin per launch.
+template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+__global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
+    __shared__ WaveScratch scratch[4];
+    WaveScratch &L = scratch[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // lanes past the last spin stay in the loop (done) for the wave operations
+    const bool live = i < P.n;
+    uint32_t sid = 0, comp = 0, fiber = kFiberNone, s0 = 0, s1 = 0;
+    float x = 0.f, y = 0.f, z = 0.f, mag = 0.f, ph = 0.f;
+    float dx = 0.f, dy = 0.f, dz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    float safe = -1.0f;
+    if (live) {
+        sid = P.spin_id ? P.spin_id[i] : i;
+        x = P.x[i]; y = P.y[i]; z = P.z[i];
+        mag = P.mag[i];
+        if (!MOMENT) ph = P.phase[i];
+        const uint32_t lab = P.label[i];
+        comp = lab & 0xFFu; fiber = (lab >> 8) & 0xFFFFu;
+        if (RNG == kRngMwc) { s0 = P.seed[2 * i]; s1 = P.seed[2 * i + 1]; }
+        if (MOMENT) {
+            dx = P.disp[i]; dy = P.disp[P.n + i]; dz = P.disp[2 * P.n + i];
+            mx = P.mom[i]; my = P.mom[P.n + i]; mz = P.mom[2 * P.n + i];
+            cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
+        }
+        if (P.cell_clear) {
+            uint32_t qx, qy, qz;
+            cell_of<1>(P, x, y, z, qx, qy, qz);
+            safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
+        }
+    }
+    uint32_t it = live ? 0u : iters;
+    // the pending segment of the current step and its permeation interval
+    bool pending = false, have_u = true, refresh = false;
+    float ox = 0.f, oy = 0.f, oz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+    float u = 0.f, umax = 1.f, umin = 0.f;
+    uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0;
+    while (__ballot(it < iters) != 0ull) {
+        bool complete = false;
+        if (it < iters && !pending) {
+            const uint64_t gstep = P.step0 + it;
+            float n0, n1, n2;
+            have_u = true;
+            if (RNG == kRngInject) {
+                const float4 q = P.inject[(size_t)it * P.n + sid];
+                n0 = q.x; n1 = q.y; n2 = q.z; u = q.w;
+            } else if (RNG == kRngMwc) {
+                mwc_randn(s0, s1, n0, n1, n2, u);
+            } else {
+                philox_normals(P.key0, P.key1, sid, gstep, n0, n1, n2);
+                have_u = false;
+            }
+            const float sd = comp == 0 ? P.std_dev[0] : comp == 1 ? P.std_dev[1]
+                           : comp == 2 ? P.std_dev[2] : P.std_dev[3];
+            ox = x; oy = y; oz = z;
+            px = fmaf(n0, sd, x); py = fmaf(n1, sd, y); pz = fmaf(n2, sd, z);
+            first = last = excluded = kNone;
+            ncoll = 0;
+            umax = 1.0f; umin = 0.0f;
+            pending = true;
+            refresh = true;
+            if (P.cell_clear) {
+                // no triangle lies within `safe` of the path walked since the
+                // last check, so a shorter step cannot hit one
+                const float len =
+                    sd * __builtin_amdgcn_sqrtf(fmaf(n0, n0, fmaf(n1, n1, n2 * n2)));
+                if (len < safe) {
+                    safe -= len;
+                    pending = false;
+                    refresh = false;
+                    complete = true;
+                }
+            }
+        }
+        const unsigned long long key =
+            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded);
+        if (pending) {
+            if (key == ~0ull) {
+                pending = false;
+                complete = true;
+            } else {
+                const uint32_t idx = (uint32_t)key;
+                Hit h;
+                test_mt_crossing(P.tri_mt, idx, ox, oy, oz, px, py, pz, h);
+                if (ncoll == 0) first = idx;
+                last = idx;
+                ncoll += 1;
+                if (RNG == kRngPhilox && !have_u) {
+                    u = philox_uniform(P.key0, P.key1, sid, P.step0 + it);
+                    have_u = true;
+                }
+                const float thr = umax - (umax - umin) * P.perm;
+                float ex = px, ey = py, ez = pz;
+                if (u <= thr) {
+                    reflect<1>(P, ox, oy, oz, px, py, pz, h, ex, ey, ez);
+                    umax = thr;
+                } else {
+                    umin = thr;
+                    transit<1>(P, tri_info_of(P, idx, 1), comp, fiber);
+                }
+                ox = h.px; oy = h.py; oz = h.pz;
+                px = ex; py = ey; pz = ez;
+                excluded = idx;
+                if (ncoll == 4096) {
+                    atomicOr(P.flags, 1u);
+                    pending = false;
+                    complete = true;
+                }
+            }
+        }
+        if (complete) {
+            if (!PERIODIC) {
+                if (px > P.clamp_hi) px = P.clamp_hi; else if (px < P.clamp_lo) px = P.clamp_lo;
+                if (py > P.clamp_hi) py = P.clamp_hi; else if (py < P.clamp_lo) py = P.clamp_lo;
+                if (pz > P.clamp_hi) pz = P.clamp_hi; else if (pz < P.clamp_lo) pz = P.clamp_lo;
+            }
+            if (MOMENT) {
+                dx += px - x; dy += py - y; dz += pz - z;
+            }
+            if (PERIODIC) {
+                const float wx = px, wy = py, wz = pz;
+                if (px >= 1.0f) px -= 2.0f; else if (px < -1.0f) px += 2.0f;
+                if (py >= 1.0f) py -= 2.0f; else if (py < -1.0f) py += 2.0f;
+                if (pz >= 1.0f) pz -= 2.0f; else if (pz < -1.0f) pz += 2.0f;
+                refresh |= (px != wx) | (py != wy) | (pz != wz);
+            }
+            x = px; y = py; z = pz;
+            if (P.cell_clear && refresh) {
+                uint32_t qx, qy, qz;
+                cell_of<1>(P, x, y, z, qx, qy, qz);
+                safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
+            }
+            const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
+                           : comp == 2 ? P.decay[2] : P.decay[3];
+            mag = fmaf(-mag, dc, mag);
+            if (!MOMENT) ph += (P.gx * x + P.gy * y + P.gz * z) * P.pc;
+            if (MOMENT && P.weight != 0.0f) {
+                float yv, tv;
+                yv = fmaf(P.weight, dx, -cx); tv = mx + yv; cx = (tv - mx) - yv; mx = tv;
+                yv = fmaf(P.weight, dy, -cy); tv = my + yv; cy = (tv - my) - yv; my = tv;
+                yv = fmaf(P.weight, dz, -cz); tv = mz + yv; cz = (tv - mz) - yv; mz = tv;
+            }
+            if (TRACE) {
+                uint32_t qx, qy, qz;
+                cell_of<1>(P, x, y, z, qx, qy, qz);
+                uint32_t *tr = P.trace + ((size_t)it * P.n + sid) * 5;
+                tr[0] = first; tr[1] = last; tr[2] = ncoll;
+                tr[3] = cell_hash(P, qx, qy, qz);
+                tr[4] = comp | (fiber << 8);
+            }
+            ++it;
+        }
+    }
+    if (live) {
+        P.x[i] = x; P.y[i] = y; P.z[i] = z;
+        P.mag[i] = mag;
+        if (!MOMENT) P.phase[i] = ph;
+        P.label[i] = comp | (fiber << 8);
+        if (RNG == kRngMwc) { P.seed[2 * i] = s0; P.seed[2 * i + 1] = s1; }
+        if (MOMENT) {
+            P.disp[i] = dx; P.disp[P.n + i] = dy; P.disp[2 * P.n + i] = dz;
+            P.mom[i] = mx; P.mom[P.n + i] = my; P.mom[2 * P.n + i] = mz;
+            P.momc[i] = cx; P.momc[P.n + i] = cy; P.momc[2 * P.n + i] = cz;
+        }
+    }
+}
+
+template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
+    dim3 grid((p.n + 255) / 256), block(256);
+    if constexpr (MODE == 0)
+        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
+    else
+        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
+}
+
 template <int MODE, bool PERIODIC, int RNG>
 static hipError_t launch3(const StepParams &p, uint32_t iters, bool trace, bool moment,
                           hipStream_t s) {
-    dim3 grid((p.n + 255) / 256), block(256);
     if (trace) {
-        if (moment) hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, true, true>), grid, block, 0, s, p, iters);
-        else hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, true, false>), grid, block, 0, s, p, iters);
+        if (moment) launch_one<MODE, PERIODIC, RNG, true, true>(p, iters, s);
+        else launch_one<MODE, PERIODIC, RNG, true, false>(p, iters, s);
     } else {
-        if (moment) hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, false, true>), grid, block, 0, s, p, iters);
-        else hipLaunchKernelGGL((k_step<MODE, PERIODIC, RNG, false, false>), grid, block, 0, s, p, iters);
+        if (moment) launch_one<MODE, PERIODIC, RNG, false, true>(p, iters, s);
+        else launch_one<MODE, PERIODIC, RNG, false, false>(p, iters, s);
     }
     return hipGetLastError();
 }
