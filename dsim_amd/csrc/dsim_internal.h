@@ -24,6 +24,7 @@ struct StepParams {
     const uint32_t *cell_start;
     const uint32_t *cell_tri;
     const float4 *cell_plane;  // plane (n, n.v0) of cell_tri[k], cell-major
+    uint32_t n_entries;        // length of cell_tri and cell_plane
     const float *cell_clear;   // per-cell distance lower bound, or null (no skipping)
     const uint8_t *fiber_myelin;
     uint32_t n_fibers;
@@ -54,7 +55,7 @@ struct StepParams {
     uint64_t step0;
     const float4 *inject;  // [iters][n]
     uint32_t *trace;       // [iters][n][5]
-    uint32_t *flags;       // device status word (bit 0: collision loop guard hit)
+    uint32_t *flags;       // status word: bit 0 collision loop guard hit, bit 2 entry out of range
 };
 
 struct LabelParams {

@@ -206,15 +206,15 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
             for (uint64_t c = nc * w / nw; c < nc * (w + 1) / nw; ++c) {
                 int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
                 int64_t idx[3] = {ix, iy, iz};
+                // Only triangles registered in the grid are ever tested, and the
+                // step kernels use the bound only for positions inside the grid,
+                // so the grid's own edges do not limit it.
                 double b0[3], b1[3];
                 double best = 1e30;
                 for (int k = 0; k < 3; ++k) {
                     b0[k] = (double)lo[k] + (double)idx[k] * cell[k];
                     b1[k] = b0[k] + cell[k];
-                    double glo = (double)lo[k], ghi = (double)lo[k] + (double)D[k] * cell[k];
-                    best = std::min(best, std::min(b0[k] - glo, ghi - b1[k]));
                 }
-                best = std::max(best, 0.0);
                 for (int64_t r = 0;; ++r) {
                     if (r > 0 && (double)(r - 1) * hmin >= best) break;
                     if (r > D[0] && r > D[1] && r > D[2]) break;

@@ -7,6 +7,7 @@
 // physical-mode code spells its fused multiply-adds out with fmaf().
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/warp/warp_scan.hpp>
 
 #include <cstdint>
 
@@ -43,13 +44,15 @@ __device__ __forceinline__ void mwc_randn(uint32_t &s0, uint32_t &s1, float &n1,
     n3 = r2 * cosf(p2);
 }
 
-// Philox4x32-10 (Salmon et al., SC'11).
+// Philox4x32-10 (Salmon et al., SC'11).  The 32x32->64 products compile to
+// one v_mad_u64_u32 each.
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1,
+                       (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -60,25 +63,24 @@ __device__ __forceinline__ float u32_unit(uint32_t x) {
     return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f;
 }
 
-// Gaussian normals from one Philox block with the hardware transcendental
-// units: v_log_f32 is log2, v_cos_f32/v_sin_f32 take revolutions, so
-// cos(2*pi*u) is one instruction for u in (0,1).
-__device__ __forceinline__ void philox_normals(uint32_t k0, uint32_t k1, uint32_t spin,
-                                               uint64_t step, float &n0, float &n1, float &n2) {
-    uint4 o = philox(make_uint4(spin, (uint32_t)step, (uint32_t)(step >> 32), 0u), k0, k1);
-    float u1 = u32_unit(o.x), u2 = u32_unit(o.y), u3 = u32_unit(o.z), u4 = u32_unit(o.w);
+// One Philox block per spin and step (counter: spin, step lo, step hi, 0):
+// Gaussian normals from the top 23 bits of each word with the hardware
+// transcendental units (v_log_f32 is log2, v_cos_f32/v_sin_f32 take
+// revolutions, so cos(2*pi*u) is one instruction for u in (0,1)), and the
+// permeation uniform from 23 of the low bits.
+__device__ __forceinline__ void philox_draw(uint32_t k0, uint32_t k1, uint32_t spin,
+                                            uint64_t step, float &n0, float &n1, float &n2,
+                                            float &u) {
+    const uint4 o = philox(make_uint4(spin, (uint32_t)step, (uint32_t)(step >> 32), 0u), k0, k1);
+    const float u1 = u32_unit(o.x), u2 = u32_unit(o.y), u3 = u32_unit(o.z), u4 = u32_unit(o.w);
     const float m2ln2 = -1.3862943611198906f;  // -2 ln 2
-    float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1));
-    float r2 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u3));
+    const float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1));
+    const float r2 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u3));
     n0 = r1 * __builtin_amdgcn_cosf(u2);
     n1 = r1 * __builtin_amdgcn_sinf(u2);
     n2 = r2 * __builtin_amdgcn_cosf(u4);
-}
-
-__device__ __forceinline__ float philox_uniform(uint32_t k0, uint32_t k1, uint32_t spin,
-                                                uint64_t step) {
-    uint4 o = philox(make_uint4(spin, (uint32_t)step, (uint32_t)(step >> 32), 1u), k0, k1);
-    return u32_unit(o.x);
+    const uint32_t m = (o.x & 511u) | ((o.y & 511u) << 9) | ((o.z & 31u) << 18);
+    u = ((float)m + 0.5f) * 1.1920928955078125e-07f;
 }
 
 // ------------------------------------------------------------------------ //
@@ -319,10 +321,9 @@ __device__ __forceinline__ uint32_t tri_info_of(const StepParams &P, uint32_t t,
 // collDetectRectGrid, spinKernelCpu.cpp:198-301.  MODE 0 stops at the first
 // cell (x-major walk) holding a hit; MODE 1 visits the whole cell range and
 // keeps the nearest hit (t, then triangle index).
-template <int MODE, int RNG>
+template <int MODE>
 __device__ __forceinline__ void collide(const StepParams &P, float ox, float oy, float oz,
-                                        float &px, float &py, float &pz, float &u, bool &have_u,
-                                        uint32_t spin, uint64_t gstep, uint32_t &comp,
+                                        float &px, float &py, float &pz, float u, uint32_t &comp,
                                         uint32_t &fiber, uint32_t &first, uint32_t &last,
                                         uint32_t &ncoll) {
     float ex = px, ey = py, ez = pz;
@@ -367,10 +368,6 @@ __device__ __forceinline__ void collide(const StepParams &P, float ox, float oy,
         if (ncoll == 0) first = best.idx;
         last = best.idx;
         ncoll += 1;
-        if (RNG == kRngPhilox && !have_u) {
-            u = philox_uniform(P.key0, P.key1, spin, gstep);
-            have_u = true;
-        }
         float thr = umax - (umax - umin) * P.perm;
         if (u <= thr) {
             reflect<MODE>(P, ox, oy, oz, px, py, pz, best, ex, ey, ez);
@@ -387,14 +384,32 @@ __device__ __forceinline__ void collide(const StepParams &P, float ox, float oy,
     px = ex; py = ey; pz = ez;
 }
 
-// Margin kept below the cell clearance: far larger than the rounding of a
-// step (coordinates are O(1), ulp 1.2e-7), so a skipped step is one the full
-// search would also have found free.
+// Clearance shortcut.  The clearance of a cell bounds from below the distance
+// from any point of the cell to any triangle registered in the grid, so a
+// spin inside the cell can walk a path shorter than that without a search
+// finding a hit.  Two allowances keep the float arithmetic on the safe side:
+// kClearMargin for a position that rounds into the cell from just outside it,
+// and kStepSlack per step for the rounding of the new position (half an ulp
+// per coordinate at |x| < 2, 1.1e-7 as a vector) and of the step length.
 constexpr float kClearMargin = 1e-5f;
+constexpr float kStepSlack = 4e-7f;
+
+// Clearance budget of a spin at (x, y, z): the cell's clearance less the
+// margin, or -1 (no shortcut) outside the grid, where lookups clamp to an
+// edge cell that does not contain the position.
+__device__ __forceinline__ float clearance_at(const StepParams &P, float x, float y, float z) {
+    const float qx = floorf((x - P.lo[0]) * P.inv_cell[0]);
+    const float qy = floorf((y - P.lo[1]) * P.inv_cell[1]);
+    const float qz = floorf((z - P.lo[2]) * P.inv_cell[2]);
+    if (!(qx >= 0.0f && qx < (float)P.dims[0] && qy >= 0.0f && qy < (float)P.dims[1] &&
+          qz >= 0.0f && qz < (float)P.dims[2]))
+        return -1.0f;
+    return P.cell_clear[cell_hash(P, (uint32_t)qx, (uint32_t)qy, (uint32_t)qz)] - kClearMargin;
+}
 
 // ------------------------------------------------------------------------ //
-// Step kernel: cpuIntegrate (spinKernelCpu.cpp:430-499) for `iters` steps
-// per launch with the spin's state held in registers.
+// Reference-mode step kernel: cpuIntegrate (spinKernelCpu.cpp:430-499) for
+// `iters` steps per launch with the spin's state held in registers.
 // ------------------------------------------------------------------------ //
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
@@ -415,25 +430,16 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
         mx = P.mom[i]; my = P.mom[P.n + i]; mz = P.mom[2 * P.n + i];
         cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
     }
-    // distance the spin may still travel without any triangle in reach
-    float safe = -1.0f;
-    if (MODE == 1 && P.cell_clear) {
-        uint32_t qx, qy, qz;
-        cell_of<1>(P, x, y, z, qx, qy, qz);
-        safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
-    }
     for (uint32_t it = 0; it < iters; ++it) {
         const uint64_t gstep = P.step0 + it;
         float n0, n1, n2, u = 0.0f;
-        bool have_u = true;
         if (RNG == kRngInject) {
             float4 q = P.inject[(size_t)it * P.n + sid];
             n0 = q.x; n1 = q.y; n2 = q.z; u = q.w;
         } else if (RNG == kRngMwc) {
             mwc_randn(s0, s1, n0, n1, n2, u);
         } else {
-            philox_normals(P.key0, P.key1, sid, gstep, n0, n1, n2);
-            have_u = false;
+            philox_draw(P.key0, P.key1, sid, gstep, n0, n1, n2, u);
         }
         const float sd = comp == 0 ? P.std_dev[0] : comp == 1 ? P.std_dev[1]
                        : comp == 2 ? P.std_dev[2] : P.std_dev[3];
@@ -445,22 +451,7 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
             px = fmaf(n0, sd, x); py = fmaf(n1, sd, y); pz = fmaf(n2, sd, z);
         }
         uint32_t first = kNone, last = kNone, ncoll = 0;
-        bool refresh = false;
-        if (MODE == 1 && P.cell_clear) {
-            // No triangle lies within `safe` of the path walked since the last
-            // check, so a shorter step cannot hit one: skip the search.
-            const float len = sd * __builtin_amdgcn_sqrtf(fmaf(n0, n0, fmaf(n1, n1, n2 * n2)));
-            if (len < safe) {
-                safe -= len;
-            } else {
-                collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber,
-                                   first, last, ncoll);
-                refresh = true;
-            }
-        } else {
-            collide<MODE, RNG>(P, ox, oy, oz, px, py, pz, u, have_u, sid, gstep, comp, fiber,
-                               first, last, ncoll);
-        }
+        collide<MODE>(P, ox, oy, oz, px, py, pz, u, comp, fiber, first, last, ncoll);
         if (!PERIODIC) {
             // spinKernelCpu.cpp:469-474
             if (px > P.clamp_hi) px = P.clamp_hi; else if (px < P.clamp_lo) px = P.clamp_lo;
@@ -471,18 +462,11 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
             dx += px - ox; dy += py - oy; dz += pz - oz;
         }
         if (PERIODIC) {
-            const float wx = px, wy = py, wz = pz;
             if (px >= 1.0f) px -= 2.0f; else if (px < -1.0f) px += 2.0f;
             if (py >= 1.0f) py -= 2.0f; else if (py < -1.0f) py += 2.0f;
             if (pz >= 1.0f) pz -= 2.0f; else if (pz < -1.0f) pz += 2.0f;
-            refresh |= (px != wx) | (py != wy) | (pz != wz);
         }
         x = px; y = py; z = pz;
-        if (MODE == 1 && P.cell_clear && refresh) {
-            uint32_t qx, qy, qz;
-            cell_of<1>(P, x, y, z, qx, qy, qz);
-            safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
-        }
         const float t2 = comp == 0 ? P.t2[0] : comp == 1 ? P.t2[1] : comp == 2 ? P.t2[2] : P.t2[3];
         const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
                        : comp == 2 ? P.decay[2] : P.decay[3];
@@ -517,8 +501,7 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
 }
 
 // ------------------------------------------------------------------------ //
-// Physical-mode walk: the same per-spin arithmetic as k_step<1,...>, laid
-// out for 64-wide wavefronts.
+// Physical-mode walk, laid out for 64-wide wavefronts.
 //
 // Each lane runs a small state machine: it either starts its next step (draw
 // normals, propose the end point, take the clearance shortcut if it applies)
@@ -527,20 +510,29 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
 // together, so a lane whose step ended without a hit starts its next step
 // while another lane is still reflecting.
 //
-// The search is compacted over the wave: each pending lane lists the CSR
-// ranges of the cells its segment's bounding box covers, a wave prefix sum
-// numbers the (lane, candidate) pairs, and the wave tests 64 pairs per pass,
-// whatever lane they belong to.  The nearest hit per lane is an LDS atomic
-// minimum over (t bits, triangle index): for t > 0 the float bits order like
-// the values, so this is the same (t, then index) order as the per-lane scan.
+// The search is compacted over the wave in two passes.  Each pending lane
+// lists the CSR ranges of the cells its segment's bounding box covers, and a
+// wave prefix sum numbers the (lane, grid entry) pairs.  The plane pass tests
+// 64 pairs per trip, whatever lane they belong to, against the entry's plane
+// record and queues the pairs whose plane the segment crosses; the triangle
+// pass runs Moller-Trumbore on the queue, 64 pairs per trip.  The nearest hit
+// per lane is an LDS atomic minimum over (t bits, triangle index): for t > 0
+// the float bits order like the values, so this is the same (t, then index)
+// order as a scan of the lane's cells in any order; the pair holding the
+// minimum then writes its hit point for the lane.
 // ------------------------------------------------------------------------ //
-constexpr uint32_t kWaveSegs = 256;  // cell ranges listed per wave and search round
+constexpr uint32_t kWaveSegs = 256;   // cell ranges listed per wave and search round
+constexpr uint32_t kWaveQueue = 128;  // plane crossings awaiting the triangle test
 
 struct WaveScratch {
-    uint2 seg[kWaveSegs];  // (first entry - lane-local offset, lane-local end offset)
+    uint2 seg[kWaveSegs];  // (first entry - lane-local pair number, lane-local end)
     float4 o[64];          // segment start; .w = excluded triangle
     float4 p[64];          // segment end
+    float4 hit[64];        // nearest hit point; .w = hit type
+    uint2 meta[64];        // (first pair number, first slot in seg)
     unsigned long long best[64];
+    uint32_t map[64];      // lane + 1 at the pass position where its pairs start
+    uint2 queue[kWaveQueue];  // (grid entry, lane)
 };
 
 // Orders this wave's LDS accesses: a wave's LDS instructions complete in
@@ -551,29 +543,72 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Exclusive prefix sum over the wave; every lane must be active.
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane, uint32_t &total) {
-    uint32_t x = v;
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    total = __shfl(x, 63, 64);
-    return x - v;
+using WaveScan = rocprim::warp_scan<uint32_t, 64>;  // DPP row shifts and broadcasts
+
+// Inclusive prefix sum / maximum over the wave; every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+    typename WaveScan::storage_type st;
+    uint32_t out;
+    WaveScan().inclusive_scan(v, out, st, rocprim::plus<uint32_t>());
+    return out;
+}
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+    typename WaveScan::storage_type st;
+    uint32_t out;
+    WaveScan().inclusive_scan(v, out, st, rocprim::maximum<uint32_t>());
+    return out;
+}
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
     return ((unsigned long long)__float_as_uint(t) << 32) | tri;
 }
 
-// One search round: nearest hit of each searching lane's segment o-p,
-// excluding `excluded`, as hit_key(t, triangle), or ~0 when there is none.
-// Called by all 64 lanes of the wave together.
-__device__ __forceinline__ unsigned long long wave_search(const StepParams &P, WaveScratch &L,
-                                                          uint32_t lane, bool search, float ox,
-                                                          float oy, float oz, float px, float py,
-                                                          float pz, uint32_t excluded) {
+// Triangle pass over queue[0, qlen): nearest hit per lane into best[] and its
+// point into hit[].  Wave-uniform qlen.
+__device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScratch &L,
+                                                   uint32_t lane, uint32_t qlen) {
+    for (uint32_t qb = 0; qb < qlen; qb += 64) {
+        wave_lds_sync();
+        const uint32_t q = qb + lane;
+        unsigned long long key = ~0ull;
+        uint32_t owner = 0;
+        Hit h;
+        h.type = 0;
+        if (q < qlen) {
+            const uint2 e = L.queue[q];
+            owner = e.y;
+            const uint32_t t = P.cell_tri[e.x];
+            const float4 so = L.o[owner], sp = L.p[owner];
+            if (t != __float_as_uint(so.w)) {
+                test_mt_crossing(P.tri_mt, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
+                if (h.type) {
+                    key = hit_key(h.d, t);
+                    atomicMin(&L.best[owner], key);
+                }
+            }
+        }
+        wave_lds_sync();
+        // the pair holding the lane's minimum so far records its hit point; a
+        // later pass that lowers the minimum overwrites it
+        if (h.type && L.best[owner] == key)
+            L.hit[owner] = make_float4(h.px, h.py, h.pz, __uint_as_float(h.type));
+    }
+    wave_lds_sync();
+}
+
+// One search round, called by all 64 lanes of the wave together.  For each
+// lane with `search` set, either finds the nearest hit of the segment o-p
+// (excluding `excluded`): returns true with key = hit_key(t, triangle) (or ~0
+// when there is none) and the hit point and type in `hit`; or returns false
+// when the wave's cell list is full this round (the lane searches again).
+__device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L, uint32_t lane,
+                                            bool search, float ox, float oy, float oz, float px,
+                                            float py, float pz, uint32_t excluded,
+                                            unsigned long long &key, float4 &hit) {
     uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0, ncell = 0;
     if (search) {
         uint32_t sx, sy, sz, tx, ty, tz;
@@ -584,10 +619,13 @@ __device__ __forceinline__ unsigned long long wave_search(const StepParams &P, W
         z0 = min(sz, tz); z1 = max(sz, tz);
         ncell = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
     }
-    uint32_t nseg_total;
-    const uint32_t soff = wave_excl_scan(min(ncell, kWaveSegs + 1), lane, nseg_total);
-    // lanes whose cells do not fit the wave's list search on their own below
-    const bool coop = search && soff + ncell <= kWaveSegs;
+    // a lane whose box alone overflows the list searches on its own; one that
+    // does not fit behind the lanes before it waits for the next round
+    const bool alone = search && ncell > kWaveSegs;
+    const uint32_t nreq = (search && !alone) ? ncell : 0u;
+    const uint32_t send = wave_incl_sum(nreq);
+    const uint32_t soff = send - nreq;
+    const bool coop = nreq > 0 && send <= kWaveSegs;
     uint32_t cnt = 0;
     if (coop) {
         uint32_t cx = x0, cy = y0, cz = z0;
@@ -611,58 +649,73 @@ __device__ __forceinline__ unsigned long long wave_search(const StepParams &P, W
                 }
             }
         }
+    }
+    const uint32_t cend = wave_incl_sum(cnt);
+    const uint32_t off = cend - cnt;
+    const uint32_t total = __builtin_amdgcn_readlane(cend, 63);
+    if (coop) {
         L.o[lane] = make_float4(ox, oy, oz, __uint_as_float(excluded));
         L.p[lane] = make_float4(px, py, pz, 0.0f);
+        L.meta[lane] = make_uint2(off, soff);
         L.best[lane] = ~0ull;
     }
-    uint32_t total;
-    const uint32_t off = wave_excl_scan(cnt, lane, total);
-    wave_lds_sync();
+    uint32_t qlen = 0;  // wave-uniform
     for (uint32_t base = 0; base < total; base += 64) {
+        L.map[lane] = 0;
+        wave_lds_sync();
+        if (cnt && off < base + 64 && off + cnt > base) L.map[off > base ? off - base : 0] = lane + 1;
+        wave_lds_sync();
+        // owner of pair base+lane: the last lane whose pairs start at or
+        // before it (position 0 always holds the lane covering `base`)
+        const uint32_t owner = wave_incl_max(L.map[lane]) - 1;
         const uint32_t j = base + lane;
-        // owner of pair j: the last lane whose range starts at or before j
-        uint32_t owner = 0, obase = 0;
-#pragma unroll
-        for (uint32_t st = 32; st; st >>= 1) {
-            const uint32_t c = owner + st;
-            const uint32_t v = __shfl(off, (int)c, 64);
-            if (v <= j) { owner = c; obase = v; }
-        }
-        const uint32_t oseg = __shfl(soff, (int)owner, 64);
+        bool cross = false;
+        uint32_t k = 0;
         if (j < total) {
-            const uint32_t r = j - obase;
-            uint32_t s = oseg;
+            const uint2 m = L.meta[owner];
+            const uint32_t r = j - m.x;
+            uint32_t s = m.y;
             uint2 e = L.seg[s];
-            while (r >= e.y) e = L.seg[++s];
-            const uint32_t k = e.x + r;
-            const float4 so = L.o[owner], sp = L.p[owner];
-            if (crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z)) {
-                const uint32_t t = P.cell_tri[k];
-                if (t != __float_as_uint(so.w)) {
-                    Hit h;
-                    test_mt_crossing(P.tri_mt, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
-                    if (h.type) atomicMin(&L.best[owner], hit_key(h.d, t));
-                }
+            while (r >= e.y && s + 1 < kWaveSegs) e = L.seg[++s];
+            k = e.x + r;
+            if (k >= P.n_entries) {  // cannot happen when the numbering is right
+                atomicOr(P.flags, 4u);
+                k = 0;
             }
+            const float4 so = L.o[owner], sp = L.p[owner];
+            cross = crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z);
+        }
+        const unsigned long long bal = __ballot(cross);
+        if (cross) L.queue[qlen + lane_rank(bal)] = make_uint2(k, owner);
+        qlen += (uint32_t)__popcll(bal);
+        if (qlen > kWaveQueue - 64) {
+            wave_triangle_pass(P, L, lane, qlen);
+            qlen = 0;
         }
     }
+    if (qlen) wave_triangle_pass(P, L, lane, qlen);
     wave_lds_sync();
-    unsigned long long key = ~0ull;
+    key = ~0ull;
     if (coop) {
         key = L.best[lane];
-    } else if (search) {
+        if (key != ~0ull) hit = L.hit[lane];
+    } else if (alone) {
         Hit best;
         best.type = 0;
         best.idx = kNone;
         best.d = 2.0f;
+        best.px = best.py = best.pz = 0.0f;
         for (uint32_t cz = z0; cz <= z1; ++cz)
             for (uint32_t cy = y0; cy <= y1; ++cy)
                 for (uint32_t cx = x0; cx <= x1; ++cx)
                     cell_test<1>(P, cell_hash(P, cx, cy, cz), excluded, ox, oy, oz, px, py, pz,
                                  best);
-        if (best.type) key = hit_key(best.d, best.idx);
+        if (best.type) {
+            key = hit_key(best.d, best.idx);
+            hit = make_float4(best.px, best.py, best.pz, __uint_as_float(best.type));
+        }
     }
-    return key;
+    return coop || alone;
 }
 
 // cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode, `iters` steps
@@ -692,32 +745,25 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
             mx = P.mom[i]; my = P.mom[P.n + i]; mz = P.mom[2 * P.n + i];
             cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
         }
-        if (P.cell_clear) {
-            uint32_t qx, qy, qz;
-            cell_of<1>(P, x, y, z, qx, qy, qz);
-            safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
-        }
+        if (P.cell_clear) safe = clearance_at(P, x, y, z);
     }
     uint32_t it = live ? 0u : iters;
     // the pending segment of the current step and its permeation interval
-    bool pending = false, have_u = true, refresh = false;
+    bool pending = false, refresh = false;
     float ox = 0.f, oy = 0.f, oz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     float u = 0.f, umax = 1.f, umin = 0.f;
     uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0;
     while (__ballot(it < iters) != 0ull) {
         bool complete = false;
         if (it < iters && !pending) {
-            const uint64_t gstep = P.step0 + it;
             float n0, n1, n2;
-            have_u = true;
             if (RNG == kRngInject) {
                 const float4 q = P.inject[(size_t)it * P.n + sid];
                 n0 = q.x; n1 = q.y; n2 = q.z; u = q.w;
             } else if (RNG == kRngMwc) {
                 mwc_randn(s0, s1, n0, n1, n2, u);
             } else {
-                philox_normals(P.key0, P.key1, sid, gstep, n0, n1, n2);
-                have_u = false;
+                philox_draw(P.key0, P.key1, sid, P.step0 + it, n0, n1, n2, u);
             }
             const float sd = comp == 0 ? P.std_dev[0] : comp == 1 ? P.std_dev[1]
                            : comp == 2 ? P.std_dev[2] : P.std_dev[3];
@@ -732,7 +778,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
                 // no triangle lies within `safe` of the path walked since the
                 // last check, so a shorter step cannot hit one
                 const float len =
-                    sd * __builtin_amdgcn_sqrtf(fmaf(n0, n0, fmaf(n1, n1, n2 * n2)));
+                    sd * __builtin_amdgcn_sqrtf(fmaf(n0, n0, fmaf(n1, n1, n2 * n2))) + kStepSlack;
                 if (len < safe) {
                     safe -= len;
                     pending = false;
@@ -741,23 +787,23 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
                 }
             }
         }
-        const unsigned long long key =
-            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded);
-        if (pending) {
+        unsigned long long key;
+        float4 hp;
+        const bool searched =
+            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, key, hp);
+        if (pending && searched) {
             if (key == ~0ull) {
                 pending = false;
                 complete = true;
             } else {
                 const uint32_t idx = (uint32_t)key;
                 Hit h;
-                test_mt_crossing(P.tri_mt, idx, ox, oy, oz, px, py, pz, h);
+                h.px = hp.x; h.py = hp.y; h.pz = hp.z;
+                h.type = __float_as_uint(hp.w);
+                h.idx = idx;
                 if (ncoll == 0) first = idx;
                 last = idx;
                 ncoll += 1;
-                if (RNG == kRngPhilox && !have_u) {
-                    u = philox_uniform(P.key0, P.key1, sid, P.step0 + it);
-                    have_u = true;
-                }
                 const float thr = umax - (umax - umin) * P.perm;
                 float ex = px, ey = py, ez = pz;
                 if (u <= thr) {
@@ -794,11 +840,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
                 refresh |= (px != wx) | (py != wy) | (pz != wz);
             }
             x = px; y = py; z = pz;
-            if (P.cell_clear && refresh) {
-                uint32_t qx, qy, qz;
-                cell_of<1>(P, x, y, z, qx, qy, qz);
-                safe = P.cell_clear[cell_hash(P, qx, qy, qz)] - kClearMargin;
-            }
+            if (P.cell_clear && refresh) safe = clearance_at(P, x, y, z);
             const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
                            : comp == 2 ? P.decay[2] : P.decay[3];
             mag = fmaf(-mag, dc, mag);

@@ -156,6 +156,10 @@ static int sync_and_check(dsim_sim *s) {
         HIP_TRY(hipMemset(s->flags.p, 0, 16));
         return fail(DSIM_ERR_OVERFLOW, "a spin exceeded 4096 collisions in one step");
     }
+    if (fl[0] & 4u) {
+        HIP_TRY(hipMemset(s->flags.p, 0, 16));
+        return fail(DSIM_ERR_STATE, "collision search numbered a grid entry out of range");
+    }
     return DSIM_OK;
 }
 
@@ -290,6 +294,7 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.cell_start = b->d_cell_start.p;
     P.cell_tri = b->d_cell_tri.p;
     P.cell_plane = b->cell_plane.p;
+    P.n_entries = (uint32_t)b->d_cell_tri.n;
     P.cell_clear = s->prm.mode == DSIM_MODE_PHYSICAL ? b->d_clearance.p : nullptr;
     P.fiber_myelin = b->fiber_myelin.p;
     P.n_fibers = b->n_fibers;

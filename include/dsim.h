@@ -127,9 +127,10 @@ int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_start, uint3
 /* Copies m_triangleHelpers (12 floats per triangle, spinSystem.cpp:612-648). */
 int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers);
 /* AABB grids only (no reference counterpart): per-cell lower bound of the
- * distance from any point of the cell to any triangle [n_cells].  The physical
- * step skips collision detection while a spin's path since its last check is
- * shorter than this clearance. */
+ * distance from any point of the cell to any triangle registered in the grid
+ * [n_cells] (1e30 when there is none).  The physical step skips collision
+ * detection while a spin that was inside the grid at its last check has
+ * walked a path shorter than this clearance. */
 int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance);
 /* Device bytes held by the substrate. */
 int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes);

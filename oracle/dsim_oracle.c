@@ -102,14 +102,16 @@ void orc_philox_step_normals(const uint32_t key[2], uint32_t spin, uint64_t step
     n[2] = r2 * cosf(TWOPI_F * u4);
 }
 
-/* Build-defined.  Permeation uniform on stream 1, drawn only when a spin hits
- * a membrane during the step. */
+/* Build-defined.  Permeation uniform of the step: the normals use the top 23
+ * bits of each word of the stream-0 block, so the 9 low bits of words 0 and 1
+ * and 5 low bits of word 2 make 23 further bits, independent of the normals. */
 float orc_philox_step_uniform(const uint32_t key[2], uint32_t spin, uint64_t step)
 {
-    uint32_t ctr[4] = {spin, (uint32_t)step, (uint32_t)(step >> 32), 1u};
+    uint32_t ctr[4] = {spin, (uint32_t)step, (uint32_t)(step >> 32), 0u};
     uint32_t o[4];
     orc_philox4x32_10(ctr, key, o);
-    return orc_u32_to_unit(o[0]);
+    uint32_t m = (o[0] & 511u) | ((o[1] & 511u) << 9) | ((o[2] & 31u) << 18);
+    return ((float)m + 0.5f) * 1.1920928955078125e-07f; /* 2^-23 */
 }
 
 /* ======================================================================== */

@@ -204,7 +204,6 @@ def test_philox_stream_matches_oracle(ref_inputs):
     dims, lo, cell = (31, 31, 7), (-1.1, -1.1, -1.1), (2.2 / 31, 2.2 / 31, 2.2 / 7)
     w, sub, sim = _pair(m, n=n, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, seed=0xABCDEF12345)
     pos, comp, fib = _spins(w, m, n, 5, plus_x=True)
-    key = (0x12345 + (0xABCDEF12345 & 0) , 0)
     key = (0xABCDEF12345 & 0xFFFFFFFF, 0xABCDEF12345 >> 32)
     st = O.State(pos, comp=comp, fiber=fib, moment=True)
     sim.set_state(pos=pos, comp=comp, fiber=fib)

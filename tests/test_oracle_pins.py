@@ -66,6 +66,22 @@ def test_philox_normals_moments():
     assert abs(np.corrcoef(n[:, 0], n[:, 1])[0, 1]) < 0.03
 
 
+def test_philox_uniform_is_the_low_bits_of_the_normals_block():
+    key, spin, step = (7, 11), 5, (1 << 33) + 9
+    o = O.philox([spin, step & 0xFFFFFFFF, step >> 32, 0], key)
+    m = (o[0] & 511) | ((o[1] & 511) << 9) | ((o[2] & 31) << 18)
+    assert O.philox_step_uniform(key, spin, step) == (m + 0.5) * 2.0 ** -23
+    # uniform on (0,1) and uncorrelated with the normals drawn from the same block
+    from scipy import stats
+    u = np.array([O.philox_step_uniform(key, i, 3) for i in range(20000)])
+    n = np.array([O.philox_step_normals(key, i, 3) for i in range(20000)])
+    assert 0.0 < u.min() and u.max() < 1.0
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    for k in range(3):
+        assert abs(np.corrcoef(u, n[:, k])[0, 1]) < 0.03
+        assert abs(np.corrcoef(u, np.abs(n[:, k]))[0, 1]) < 0.03
+
+
 # --------------------------------------------------------------------------- #
 # Triangle tests
 # --------------------------------------------------------------------------- #
