@@ -24,6 +24,7 @@ struct StepParams {
     const uint32_t *cell_start;
     const uint32_t *cell_tri;
     const float4 *cell_plane;  // plane (n, n.v0) of cell_tri[k], cell-major
+    const float4 *cell_mt;     // [3] per entry: v0 (w: triangle index), e1, e2
     uint32_t n_entries;        // length of cell_tri and cell_plane
     const float *cell_clear;   // per-cell distance lower bound, or null (no skipping)
     const uint8_t *fiber_myelin;
@@ -49,6 +50,8 @@ struct StepParams {
     float *disp;      // [3][n] unwrapped displacement
     float *mom;       // [3][n] moment
     float *momc;      // [3][n] Kahan compensation
+    uint32_t *hits;   // membrane hits per spin since the last reorder, or null
+    uint32_t hit_window;  // steps those counts cover (0: no history)
     // per call
     float gx, gy, gz, pc, weight;
     uint32_t key0, key1;

@@ -187,53 +187,79 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
 }
 
 // Per-cell clearance: a lower bound of the distance from any point of the cell
-// to any triangle, found as the smallest box-to-box distance between the cell
-// and the bounding boxes of the triangles listed in the cells around it
-// (rings of growing Chebyshev radius, stopping once a ring is farther than the
-// best found).  Geometry beyond the grid is unknown, so the distance to the
-// grid's outer face caps the value.  Rounded down to float.
+// to any triangle registered in the grid.  A breadth-first pass over the
+// 26-neighbourhood gives each cell its Chebyshev ring distance D to the
+// nearest cell holding a triangle; every triangle is then at least
+// (D - 1) * (smallest cell edge) away.  Cells within kExactRings of a
+// triangle refine that with the smallest box-to-box distance between the
+// cell and the bounding boxes of the triangles listed in the rings around it
+// (growing Chebyshev radius, stopping once a ring is farther than the best
+// found).  Rounded down to float; 1e30 when the grid holds no triangle.
 void compute_clearance(const uint32_t dims[3], const float lo[3], const float cell[3],
                        const std::vector<uint32_t> &start, const std::vector<uint32_t> &tri,
                        const std::vector<float4> &box, std::vector<float> &out) {
+    constexpr uint32_t kExactRings = 3;
+    constexpr uint32_t kFar = 0xFFFFFFFFu;
     const uint64_t nc = (uint64_t)dims[0] * dims[1] * dims[2];
     out.assign(nc, 0.0f);
     const double hmin = std::min({(double)cell[0], (double)cell[1], (double)cell[2]});
     const int64_t D[3] = {dims[0], dims[1], dims[2]};
+    // ring distance to the nearest occupied cell
+    std::vector<uint32_t> ring(nc, kFar);
+    std::vector<uint64_t> frontier, next;
+    for (uint64_t c = 0; c < nc; ++c)
+        if (start[c + 1] > start[c]) { ring[c] = 0; frontier.push_back(c); }
+    for (uint32_t r = 1; !frontier.empty(); ++r) {
+        next.clear();
+        for (uint64_t c : frontier) {
+            const int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
+            for (int64_t z = std::max<int64_t>(0, iz - 1); z <= std::min(D[2] - 1, iz + 1); ++z)
+                for (int64_t y = std::max<int64_t>(0, iy - 1); y <= std::min(D[1] - 1, iy + 1); ++y)
+                    for (int64_t x = std::max<int64_t>(0, ix - 1); x <= std::min(D[0] - 1, ix + 1); ++x) {
+                        const uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
+                        if (ring[cc] == kFar) { ring[cc] = r; next.push_back(cc); }
+                    }
+        }
+        frontier.swap(next);
+    }
     unsigned nw = std::max(1u, std::min(n_workers(), (unsigned)((nc + 4095) / 4096)));
     std::vector<std::thread> th;
     for (unsigned w = 0; w < nw; ++w) {
         th.emplace_back([&, w] {
             for (uint64_t c = nc * w / nw; c < nc * (w + 1) / nw; ++c) {
-                int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
-                int64_t idx[3] = {ix, iy, iz};
-                // Only triangles registered in the grid are ever tested, and the
-                // step kernels use the bound only for positions inside the grid,
-                // so the grid's own edges do not limit it.
-                double b0[3], b1[3];
-                double best = 1e30;
-                for (int k = 0; k < 3; ++k) {
-                    b0[k] = (double)lo[k] + (double)idx[k] * cell[k];
-                    b1[k] = b0[k] + cell[k];
-                }
-                for (int64_t r = 0;; ++r) {
-                    if (r > 0 && (double)(r - 1) * hmin >= best) break;
-                    if (r > D[0] && r > D[1] && r > D[2]) break;
-                    for (int64_t z = std::max<int64_t>(0, iz - r); z <= std::min(D[2] - 1, iz + r); ++z)
-                        for (int64_t y = std::max<int64_t>(0, iy - r); y <= std::min(D[1] - 1, iy + r); ++y)
-                            for (int64_t x = std::max<int64_t>(0, ix - r); x <= std::min(D[0] - 1, ix + r); ++x) {
-                                if (std::max({std::llabs(x - ix), std::llabs(y - iy), std::llabs(z - iz)}) != r)
-                                    continue;  // only the ring at Chebyshev radius r
-                                uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
-                                for (uint32_t k = start[cc]; k < start[cc + 1]; ++k) {
-                                    const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
-                                    double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z}, d2 = 0;
-                                    for (int q = 0; q < 3; ++q) {
-                                        double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
-                                        d2 += g * g;
+                if (ring[c] == kFar) { out[c] = 1e30f; continue; }
+                // triangles lie only in rings >= ring[c], each at least
+                // (ring - 1) * hmin from this cell
+                double best = ring[c] > 0 ? (double)(ring[c] - 1) * hmin : 0.0;
+                if (ring[c] <= kExactRings) {
+                    int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
+                    int64_t idx[3] = {ix, iy, iz};
+                    double b0[3], b1[3];
+                    for (int k = 0; k < 3; ++k) {
+                        b0[k] = (double)lo[k] + (double)idx[k] * cell[k];
+                        b1[k] = b0[k] + cell[k];
+                    }
+                    best = 1e30;
+                    for (int64_t r = ring[c];; ++r) {
+                        if (r > 0 && (double)(r - 1) * hmin >= best) break;
+                        if (r > D[0] && r > D[1] && r > D[2]) break;
+                        for (int64_t z = std::max<int64_t>(0, iz - r); z <= std::min(D[2] - 1, iz + r); ++z)
+                            for (int64_t y = std::max<int64_t>(0, iy - r); y <= std::min(D[1] - 1, iy + r); ++y)
+                                for (int64_t x = std::max<int64_t>(0, ix - r); x <= std::min(D[0] - 1, ix + r); ++x) {
+                                    if (std::max({std::llabs(x - ix), std::llabs(y - iy), std::llabs(z - iz)}) != r)
+                                        continue;  // only the ring at Chebyshev radius r
+                                    uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
+                                    for (uint32_t k = start[cc]; k < start[cc + 1]; ++k) {
+                                        const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
+                                        double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z}, d2 = 0;
+                                        for (int q = 0; q < 3; ++q) {
+                                            double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
+                                            d2 += g * g;
+                                        }
+                                        best = std::min(best, std::sqrt(d2));
                                     }
-                                    best = std::min(best, std::sqrt(d2));
                                 }
-                            }
+                    }
                 }
                 float f = (float)best;
                 if ((double)f > best) f = std::nextafter(f, 0.0f);
@@ -426,11 +452,21 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     }
     std::vector<uint32_t> ct = S->cell_tri;
     if (ct.empty()) ct.push_back(0);  // keep a valid pointer for empty grids
-    // plane of every grid entry, stored in cell order so the prefilter of a
-    // cell's candidates reads contiguous memory
-    std::vector<float4> cp(ct.size());
-    for (size_t k = 0; k < ct.size(); ++k) cp[k] = S->cell_tri.empty() ? float4{} : mt[(size_t)ct[k] * 4];
-    if ((rc = upload(S->d_cell_tri, ct)) || (rc = upload(S->cell_plane, cp))) {
+    // Per grid entry, in cell order: the plane of its triangle (the
+    // prefilter of a cell's candidates reads contiguous memory), and the
+    // triangle's Moller-Trumbore vertices with its index in v0.w, so the test
+    // needs no second lookup through cell_tri.
+    std::vector<float4> cp(ct.size()), cm(ct.size() * 3);
+    for (size_t k = 0; k < ct.size(); ++k) {
+        if (S->cell_tri.empty()) break;
+        const float4 *r = &mt[(size_t)ct[k] * 4];
+        cp[k] = r[0];
+        cm[k * 3 + 0] = float4{r[1].x, r[1].y, r[1].z, bits_to_float(ct[k])};
+        cm[k * 3 + 1] = r[2];
+        cm[k * 3 + 2] = r[3];
+    }
+    if ((rc = upload(S->d_cell_tri, ct)) || (rc = upload(S->cell_plane, cp)) ||
+        (rc = upload(S->cell_mt, cm))) {
         delete S;
         return rc;
     }
@@ -478,6 +514,7 @@ extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helper
 extern "C" int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(DSIM_ERR_ARG, "NULL argument");
     *bytes = (uint64_t)s->tri_ref.n * 16 + s->tri_mt.n * 16 + s->tri_aabb.n * 16 + s->cell_plane.n * 16 +
+             s->cell_mt.n * 16 +
              s->d_cell_start.n * 4 + s->d_cell_tri.n * 4 + s->fiber_myelin.n;
     return DSIM_OK;
 }

@@ -68,14 +68,14 @@ struct dsim_substrate {
     uint32_t dims[3] = {0, 0, 0};
     float lo[3] = {0, 0, 0}, cell[3] = {0, 0, 0}, inv_cell[3] = {0, 0, 0};
     std::vector<float> clearance;  // per cell: lower bound of the distance to any triangle
-    dsim_host::DevBuf<float4> tri_ref, tri_mt, tri_aabb, cell_plane;
+    dsim_host::DevBuf<float4> tri_ref, tri_mt, tri_aabb, cell_plane, cell_mt;
     dsim_host::DevBuf<uint32_t> d_cell_start, d_cell_tri;
     dsim_host::DevBuf<float> d_clearance;
     dsim_host::DevBuf<uint8_t> fiber_myelin;
     int refs = 0;
     bool on_device = false;
     ~dsim_substrate() {
-        tri_ref.release(); tri_mt.release(); tri_aabb.release(); cell_plane.release();
+        tri_ref.release(); tri_mt.release(); tri_aabb.release(); cell_plane.release(); cell_mt.release();
         d_clearance.release();
         d_cell_start.release(); d_cell_tri.release(); fiber_myelin.release();
     }

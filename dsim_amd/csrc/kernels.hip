@@ -168,27 +168,12 @@ __device__ __forceinline__ bool crosses_plane(const float4 pl, float ox, float o
     return (s0 > 0.0f) != (s1 > 0.0f);
 }
 
-// Moller-Trumbore on the open segment 0<t<1, for a triangle whose plane the
-// segment crosses.
-__device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, uint32_t t,
-                                                 float ox, float oy, float oz, float px, float py,
-                                                 float pz, Hit &h);
-
-// Plane-side prefilter, then Moller-Trumbore on the open segment 0<t<1.
-__device__ __forceinline__ void test_mt(const float4 *__restrict__ M, uint32_t t, float ox,
-                                        float oy, float oz, float px, float py, float pz, Hit &h) {
+// Moller-Trumbore on the open segment 0<t<1 for triangle t with first vertex
+// v0 and edges e1, e2 (xyz), whose plane the segment crosses.
+__device__ __forceinline__ void mt_core(const float4 v0, const float4 e1, const float4 e2,
+                                        uint32_t t, float ox, float oy, float oz, float px,
+                                        float py, float pz, Hit &h) {
     h.type = 0;
-    if (!crosses_plane(M[t * 4 + 0], ox, oy, oz, px, py, pz)) return;
-    test_mt_crossing(M, t, ox, oy, oz, px, py, pz, h);
-}
-
-__device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, uint32_t t,
-                                                 float ox, float oy, float oz, float px, float py,
-                                                 float pz, Hit &h) {
-    h.type = 0;
-    const float4 v0 = M[t * 4 + 1];
-    const float4 e1 = M[t * 4 + 2];
-    const float4 e2 = M[t * 4 + 3];
     float dx = px - ox, dy = py - oy, dz = pz - oz;
     float qx_ = fmaf(dy, e2.z, -(dz * e2.y));
     float qy_ = fmaf(dz, e2.x, -(dx * e2.z));
@@ -217,6 +202,21 @@ __device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, u
         if ((u == 0.0f && v == 0.0f) || (u == 1.0f && v == 0.0f) || (u == 0.0f && v == 1.0f))
             h.type = 3;
     }
+}
+
+// The same test on the triangle's record in the MT table.
+__device__ __forceinline__ void test_mt_crossing(const float4 *__restrict__ M, uint32_t t,
+                                                 float ox, float oy, float oz, float px, float py,
+                                                 float pz, Hit &h) {
+    mt_core(M[t * 4 + 1], M[t * 4 + 2], M[t * 4 + 3], t, ox, oy, oz, px, py, pz, h);
+}
+
+// Plane-side prefilter, then Moller-Trumbore on the open segment 0<t<1.
+__device__ __forceinline__ void test_mt(const float4 *__restrict__ M, uint32_t t, float ox,
+                                        float oy, float oz, float px, float py, float pz, Hit &h) {
+    h.type = 0;
+    if (!crosses_plane(M[t * 4 + 0], ox, oy, oz, px, py, pz)) return;
+    test_mt_crossing(M, t, ox, oy, oz, px, py, pz, h);
 }
 
 template <int MODE>
@@ -522,7 +522,8 @@ __global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
 // minimum then writes its hit point for the lane.
 // ------------------------------------------------------------------------ //
 constexpr uint32_t kWaveSegs = 256;   // cell ranges listed per wave and search round
-constexpr uint32_t kWaveQueue = 128;  // plane crossings awaiting the triangle test
+constexpr uint32_t kPass = 128;       // pairs per pass: two per lane, loads in flight together
+constexpr uint32_t kWaveQueue = 192;  // plane crossings awaiting the triangle test
 
 struct WaveScratch {
     uint2 seg[kWaveSegs];  // (first entry - lane-local pair number, lane-local end)
@@ -531,7 +532,7 @@ struct WaveScratch {
     float4 hit[64];        // nearest hit point; .w = hit type
     uint2 meta[64];        // (first pair number, first slot in seg)
     unsigned long long best[64];
-    uint32_t map[64];      // lane + 1 at the pass position where its pairs start
+    uint32_t map[kPass];   // lane + 1 at the pass position where its pairs start
     uint2 queue[kWaveQueue];  // (grid entry, lane)
 };
 
@@ -567,37 +568,71 @@ __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
     return ((unsigned long long)__float_as_uint(t) << 32) | tri;
 }
 
+// Triangle test of queue entry q (cell-major record) for its lane; the
+// lane's minimum key in best[] is lowered to it on a hit.
+__device__ __forceinline__ void queue_test(const StepParams &P, WaveScratch &L, uint32_t q,
+                                           uint32_t qlen, Hit &h, uint32_t &owner,
+                                           unsigned long long &key) {
+    h.type = 0;
+    owner = 0;
+    key = ~0ull;
+    if (q >= qlen) return;
+    const uint2 e = L.queue[q];
+    owner = e.y;
+    const float4 v0 = P.cell_mt[(size_t)e.x * 3 + 0];
+    const float4 e1 = P.cell_mt[(size_t)e.x * 3 + 1];
+    const float4 e2 = P.cell_mt[(size_t)e.x * 3 + 2];
+    const uint32_t t = __float_as_uint(v0.w);
+    const float4 so = L.o[owner], sp = L.p[owner];
+    if (t == __float_as_uint(so.w)) return;
+    mt_core(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
+    if (h.type) {
+        key = hit_key(h.d, t);
+        atomicMin(&L.best[owner], key);
+    }
+}
+
 // Triangle pass over queue[0, qlen): nearest hit per lane into best[] and its
 // point into hit[].  Wave-uniform qlen.
 __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScratch &L,
                                                    uint32_t lane, uint32_t qlen) {
-    for (uint32_t qb = 0; qb < qlen; qb += 64) {
+    for (uint32_t qb = 0; qb < qlen; qb += kPass) {
         wave_lds_sync();
-        const uint32_t q = qb + lane;
-        unsigned long long key = ~0ull;
-        uint32_t owner = 0;
-        Hit h;
-        h.type = 0;
-        if (q < qlen) {
-            const uint2 e = L.queue[q];
-            owner = e.y;
-            const uint32_t t = P.cell_tri[e.x];
-            const float4 so = L.o[owner], sp = L.p[owner];
-            if (t != __float_as_uint(so.w)) {
-                test_mt_crossing(P.tri_mt, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
-                if (h.type) {
-                    key = hit_key(h.d, t);
-                    atomicMin(&L.best[owner], key);
-                }
-            }
-        }
+        Hit ha, hb;
+        uint32_t oa, ob;
+        unsigned long long ka, kb;
+        queue_test(P, L, qb + lane, qlen, ha, oa, ka);
+        queue_test(P, L, qb + 64 + lane, qlen, hb, ob, kb);
         wave_lds_sync();
         // the pair holding the lane's minimum so far records its hit point; a
         // later pass that lowers the minimum overwrites it
-        if (h.type && L.best[owner] == key)
-            L.hit[owner] = make_float4(h.px, h.py, h.pz, __uint_as_float(h.type));
+        if (ha.type && L.best[oa] == ka)
+            L.hit[oa] = make_float4(ha.px, ha.py, ha.pz, __uint_as_float(ha.type));
+        if (hb.type && L.best[ob] == kb)
+            L.hit[ob] = make_float4(hb.px, hb.py, hb.pz, __uint_as_float(hb.type));
     }
     wave_lds_sync();
+}
+
+// Plane test of pair j for its owner lane: sets k (grid entry) and whether
+// the owner's segment crosses the entry's plane.
+__device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScratch &L,
+                                             uint32_t j, uint32_t total, uint32_t owner,
+                                             uint32_t &k) {
+    k = 0;
+    if (j >= total) return false;
+    const uint2 m = L.meta[owner];
+    const uint32_t r = j - m.x;
+    uint32_t s = m.y;
+    uint2 e = L.seg[s];
+    while (r >= e.y && s + 1 < kWaveSegs) e = L.seg[++s];
+    k = e.x + r;
+    if (k >= P.n_entries) {  // cannot happen when the numbering is right
+        atomicOr(P.flags, 4u);
+        k = 0;
+    }
+    const float4 so = L.o[owner], sp = L.p[owner];
+    return crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z);
 }
 
 // One search round, called by all 64 lanes of the wave together.  For each
@@ -660,35 +695,26 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
         L.best[lane] = ~0ull;
     }
     uint32_t qlen = 0;  // wave-uniform
-    for (uint32_t base = 0; base < total; base += 64) {
+    for (uint32_t base = 0; base < total; base += kPass) {
         L.map[lane] = 0;
+        L.map[lane + 64] = 0;
         wave_lds_sync();
-        if (cnt && off < base + 64 && off + cnt > base) L.map[off > base ? off - base : 0] = lane + 1;
+        if (cnt && off < base + kPass && off + cnt > base) L.map[off > base ? off - base : 0] = lane + 1;
         wave_lds_sync();
-        // owner of pair base+lane: the last lane whose pairs start at or
+        // owner of pass position q: the last lane whose pairs start at or
         // before it (position 0 always holds the lane covering `base`)
-        const uint32_t owner = wave_incl_max(L.map[lane]) - 1;
-        const uint32_t j = base + lane;
-        bool cross = false;
-        uint32_t k = 0;
-        if (j < total) {
-            const uint2 m = L.meta[owner];
-            const uint32_t r = j - m.x;
-            uint32_t s = m.y;
-            uint2 e = L.seg[s];
-            while (r >= e.y && s + 1 < kWaveSegs) e = L.seg[++s];
-            k = e.x + r;
-            if (k >= P.n_entries) {  // cannot happen when the numbering is right
-                atomicOr(P.flags, 4u);
-                k = 0;
-            }
-            const float4 so = L.o[owner], sp = L.p[owner];
-            cross = crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z);
-        }
-        const unsigned long long bal = __ballot(cross);
-        if (cross) L.queue[qlen + lane_rank(bal)] = make_uint2(k, owner);
-        qlen += (uint32_t)__popcll(bal);
-        if (qlen > kWaveQueue - 64) {
+        const uint32_t own_a = wave_incl_max(L.map[lane]);
+        const uint32_t own_b = max(wave_incl_max(L.map[lane + 64]),
+                                   __builtin_amdgcn_readlane(own_a, 63));
+        uint32_t ka, kb;
+        const bool ca = pair_crosses(P, L, base + lane, total, own_a - 1, ka);
+        const bool cb = pair_crosses(P, L, base + 64 + lane, total, own_b - 1, kb);
+        const unsigned long long ba = __ballot(ca), bb = __ballot(cb);
+        if (ca) L.queue[qlen + lane_rank(ba)] = make_uint2(ka, own_a - 1);
+        qlen += (uint32_t)__popcll(ba);
+        if (cb) L.queue[qlen + lane_rank(bb)] = make_uint2(kb, own_b - 1);
+        qlen += (uint32_t)__popcll(bb);
+        if (qlen > kWaveQueue - kPass) {
             wave_triangle_pass(P, L, lane, qlen);
             qlen = 0;
         }
@@ -752,7 +778,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
     bool pending = false, refresh = false;
     float ox = 0.f, oy = 0.f, oz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     float u = 0.f, umax = 1.f, umin = 0.f;
-    uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0;
+    uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0, nhit = 0;
     while (__ballot(it < iters) != 0ull) {
         bool complete = false;
         if (it < iters && !pending) {
@@ -804,6 +830,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
                 if (ncoll == 0) first = idx;
                 last = idx;
                 ncoll += 1;
+                nhit += 1;
                 const float thr = umax - (umax - umin) * P.perm;
                 float ex = px, ey = py, ez = pz;
                 if (u <= thr) {
@@ -873,6 +900,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
             P.mom[i] = mx; P.mom[P.n + i] = my; P.mom[2 * P.n + i] = mz;
             P.momc[i] = cx; P.momc[P.n + i] = cy; P.momc[2 * P.n + i] = cz;
         }
+        if (P.hits) P.hits[i] += nhit;
     }
 }
 
@@ -1184,18 +1212,29 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every t
     return v;
 }
 
+// Sort key of a spin: its membrane-hit rate since the last reorder in 16
+// bands, highest first, then the Morton index of its cell.  Spins that hit
+// at like rates share a wavefront, so no lane holds the wave back for long,
+// and the busiest workgroups are dispatched first.  Resets the hit counts.
 __global__ __launch_bounds__(256) void k_sort_key(StepParams P, uint32_t *keys, uint32_t *vals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= P.n) return;
     uint32_t cx, cy, cz;
     cell_of<1>(P, P.x[i], P.y[i], P.z[i], cx, cy, cz);
-    keys[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    uint32_t band = 0;
+    if (P.hits) {
+        if (P.hit_window)
+            band = (uint32_t)min<uint64_t>(15, (uint64_t)P.hits[i] * 16u / P.hit_window);
+        P.hits[i] = 0;
+    }
+    const uint32_t morton = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    keys[i] = ((15u - band) << 28) | (morton >> 2);
     vals[i] = i;
 }
 
 hipError_t sort_temp_bytes(uint32_t n, size_t *bytes) {
     return rocprim::radix_sort_pairs(nullptr, *bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 30);
+                                     (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
 }
 
 hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *keys_out,
@@ -1206,7 +1245,7 @@ hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *ke
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, p.n,
-                                     0, 30, s);
+                                     0, 32, s);
 }
 
 __global__ __launch_bounds__(256) void k_gather(uint32_t n, const uint32_t *perm, Fields f) {

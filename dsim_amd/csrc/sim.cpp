@@ -24,7 +24,8 @@ namespace {
 enum Field {
     F_X, F_Y, F_Z, F_MAG, F_PHASE, F_LABEL, F_ID,
     F_DISP, F_MOM = F_DISP + 3, F_MOMC = F_MOM + 3, F_X0 = F_MOMC + 3,
-    F_COUNT = F_X0 + 3
+    F_HITS = F_X0 + 3,  // membrane hits since the last reorder (orders the next one)
+    F_COUNT = F_HITS + 1
 };
 
 const uint32_t kDefaultStepsPerLaunch = 256;
@@ -294,6 +295,7 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.cell_start = b->d_cell_start.p;
     P.cell_tri = b->d_cell_tri.p;
     P.cell_plane = b->cell_plane.p;
+    P.cell_mt = b->cell_mt.p;
     P.n_entries = (uint32_t)b->d_cell_tri.n;
     P.cell_clear = s->prm.mode == DSIM_MODE_PHYSICAL ? b->d_clearance.p : nullptr;
     P.fiber_myelin = b->fiber_myelin.p;
@@ -318,15 +320,19 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.spin_id = s->f(F_ID);
     P.seed = s->seed.p;
     P.disp = s->ff(F_DISP); P.mom = s->ff(F_MOM); P.momc = s->ff(F_MOMC);
+    P.hits = s->sorting() ? s->f(F_HITS) : nullptr;
     P.key0 = (uint32_t)s->prm.seed;
     P.key1 = (uint32_t)(s->prm.seed >> 32);
     P.flags = s->flags.p;
     return P;
 }
 
-// Reorders the spins by the Morton index of their cell.
+// Reorders the spins by their recent membrane-hit rate, then by the Morton
+// index of their cell (the key is built on the device).
 static int spatial_sort(dsim_sim *s) {
     dsim::StepParams P = step_params(s);
+    // hit counts cover the steps since the last reorder (none before the first)
+    P.hit_window = s->steps_since_sort < (1ull << 31) ? (uint32_t)s->steps_since_sort : 0u;
     hipError_t e = dsim::launch_sort_perm(P, s->keys_a.p, s->keys_b.p, s->vals_a.p, s->vals_b.p,
                                           s->sort_tmp.p, s->sort_tmp.n, s->stream);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "sort: %s", hipGetErrorString(e));
