@@ -86,6 +86,21 @@ def cpu_baseline(env, sample_spins, target_s, threads):
                 n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings calibrate the HBM traffic counted for the step kernel."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB three times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,12 +113,17 @@ def main():
     ap.add_argument("--cell-um", type=float, default=0.0)
     ap.add_argument("--sort", type=int, default=-1)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path is data-parallel over spins and "
-              f"shards by spin id with one FP64 signal all-reduce (see DESIGN.md); "
+              f"would shard by spin id with one FP64 signal all-reduce; "
               f"--gpus {args.gpus} is not supported in this build.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if D.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -119,7 +139,7 @@ def main():
     cell_um, sort_interval = args.cell_um or 0.25, 128 if args.sort < 0 else args.sort
     if not args.no_autotune and not args.cell_um:
         t0 = time.perf_counter()
-        best, allr = AT.tune(cfg, axons, n, sorts=(0, 128) if args.sort < 0 else (args.sort,),
+        best, allr = AT.tune(cfg, axons, n, sorts=(0, 128, 256) if args.sort < 0 else (args.sort,),
                              log=log)
         cell_um, sort_interval = best["cell_um"], best["sort_interval"]
         tuned = dict(cell_um=cell_um, sort_interval=sort_interval,

@@ -42,10 +42,11 @@ def time_steps(sim, steps, warm=64):
     return ms / 1e3
 
 
-def tune(cfg, axons, n, *, cells_um=(0.2, 0.25, 0.33, 0.5, 1.0), sorts=(0, 128),
-         steps=256, log=None):
+def tune(cfg, axons, n, *, cells_um=(0.15, 0.2, 0.25, 0.33, 0.5), sorts=(0, 128, 256),
+         steps=512, warm=320, log=None):
     """Returns (best dict, all results).  Each result: cell_um, sort_interval,
-    spin-steps/s, grid dims."""
+    spin-steps/s, grid dims.  The warm-up runs past the first reorder so that
+    the timed steps see reorders keyed by measured hit rates."""
     results = []
     for h in cells_um:
         env = S.build(cfg, cell_um=h, axons=axons)
@@ -54,7 +55,7 @@ def tune(cfg, axons, n, *, cells_um=(0.2, 0.25, 0.33, 0.5, 1.0), sorts=(0, 128),
             sub, sim = make_sim(env, n, sort_interval=srt)
             sim.init_uniform(1.0)
             sim.label(env["xmax"], env["xmax"], env["xmax"])
-            sec = time_steps(sim, steps)
+            sec = time_steps(sim, steps, warm=warm)
             rate = n * steps / sec
             r = dict(cell_um=h, sort_interval=srt, rate=rate, grid=list(env["grid"]["dims"]),
                      setup_s=time.perf_counter() - t0 - sec)

@@ -744,6 +744,22 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
     return coop || alone;
 }
 
+// Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8),
+// and each XCD has its own L2.  The spins are renumbered so that each XCD
+// takes runs of kXcdRun consecutive spin blocks: once the spins are sorted
+// (rate band, then Morton order) the blocks an XCD holds at a time cover a
+// compact region, whose triangles stay in its L2.  Blocks past the last whole
+// round of runs keep their number.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdRun = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t full = nb / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+    if (b >= full) return b;
+    const uint32_t x = b % kXcds, k = b / kXcds;  // XCD and its k-th workgroup
+    return ((k / kXcdRun) * kXcds + x) * kXcdRun + k % kXcdRun;
+}
+
 // cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode, `iters` steps
 // per spin per launch.
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
@@ -751,7 +767,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[4];
     WaveScratch &L = scratch[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     // lanes past the last spin stay in the loop (done) for the wave operations
     const bool live = i < P.n;
     uint32_t sid = 0, comp = 0, fiber = kFiberNone, s0 = 0, s1 = 0;
@@ -1213,9 +1229,10 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every t
 }
 
 // Sort key of a spin: its membrane-hit rate since the last reorder in 16
-// bands, highest first, then the Morton index of its cell.  Spins that hit
-// at like rates share a wavefront, so no lane holds the wave back for long,
-// and the busiest workgroups are dispatched first.  Resets the hit counts.
+// bands, highest first, then the Morton index of its cell.  Spins that hit at
+// like rates share a wavefront, so no lane holds the wave back for long; the
+// busiest workgroups are dispatched first; and within a band the Morton order
+// keeps neighbouring blocks on neighbouring triangles.  Resets the hit counts.
 __global__ __launch_bounds__(256) void k_sort_key(StepParams P, uint32_t *keys, uint32_t *vals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= P.n) return;
