@@ -54,9 +54,10 @@ def flops_per_spin_step(n_tri, n_coll, n_ch=3):
     return 50 + 7 * n_ch + 45 * n_tri + 21 * n_coll
 
 
-def cpu_baseline(env, sample_spins, target_s, threads):
+def cpu_baseline(env, target_s, threads):
     """The oracle (plain-C restatement, same physical-mode semantics) on a
-    bounded sample of the same workload, timed on this host's cores."""
+    bounded sample of the same workload, timed on this host's cores: a short
+    calibration run sizes the sample to about target_s seconds."""
     import oracle as O
 
     grid = env["grid"]
@@ -64,26 +65,57 @@ def cpu_baseline(env, sample_spins, target_s, threads):
                 lo=grid["lo"], cell=grid["cell"], eps=grid["eps"], std_dev=env["std_dev"],
                 t2=env["t2"], dt=env["cfg"].dt, permeability=env["permeability"], test=1,
                 traversal=1, transitions=1, boundary=1)
-    rng = np.random.default_rng(11)
-    pos = rng.uniform(-1, 1, (sample_spins, 3)).astype(np.float32)
-    comp, fib, _, _ = w.find_spins(pos, env["xmax"], env["xmax"], env["xmax"], plus_x=True)
-    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+
+    def spins(count, seed):
+        rng = np.random.default_rng(seed)
+        pos = rng.uniform(-1, 1, (count, 3)).astype(np.float32)
+        comp, fib, _, _ = w.find_spins(pos, env["xmax"], env["xmax"], env["xmax"], plus_x=True)
+        return O.State(pos, comp=comp, fiber=fib, moment=True)
+
+    # work-model counts from a short run with statistics on, then an untimed
+    # warm-up and a calibration run without them
     stats = np.zeros(3, np.uint64)
-    cal = 100
+    cal_spins, cal_steps = 20000, 200
+    st = spins(cal_spins, 11)
+    O.integrate(w, st, 100, rng=O.RNG_PHILOX, key=(5, 6), step0=0, weight=1.0,
+                n_threads=threads, stats=stats)
     t0 = time.perf_counter()
-    O.integrate(w, st, cal, rng=O.RNG_PHILOX, key=(5, 6), step0=0, weight=1.0, n_threads=threads,
-                stats=stats)
-    dt0 = time.perf_counter() - t0
-    steps = int(max(cal, min(12000, cal * target_s / max(dt0, 1e-6))))
+    O.integrate(w, st, cal_steps, rng=O.RNG_PHILOX, key=(5, 6), step0=100, weight=1.0,
+                n_threads=threads)
+    rate0 = cal_spins * cal_steps / max(time.perf_counter() - t0, 1e-6)
+    total = rate0 * target_s
+    n_spins = int(min(200000, max(cal_spins, total / 2000)))
+    steps = int(max(100, min(12000, total / n_spins)))
+    st = spins(n_spins, 12)
     t0 = time.perf_counter()
-    O.integrate(w, st, steps, rng=O.RNG_PHILOX, key=(5, 6), step0=cal, weight=1.0,
+    O.integrate(w, st, steps, rng=O.RNG_PHILOX, key=(5, 6), step0=0, weight=1.0,
                 n_threads=threads)
     el = time.perf_counter() - t0
     n_steps, n_cand, n_hit = (int(x) for x in stats)
-    return dict(value=sample_spins * steps / el, unit="spin-steps/s", cores=threads, kind="port",
-                sample=f"{sample_spins} spins x {steps} steps of the same substrate and physics "
+    return dict(value=n_spins * steps / el, unit="spin-steps/s", cores=threads, kind="port",
+                sample=f"{n_spins} spins x {steps} steps of the same substrate and physics "
                        f"(oracle, {threads} OpenMP threads, {el:.1f} s)",
                 n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walk_scale-2000axon-1e6_hbm_traffic.md")
+
+
+def committed_traffic(cell_um, sort_interval, n):
+    """HBM bytes per step launch of k_walk from the committed FETCH_SIZE /
+    WRITE_SIZE summary (rocprofv3 --pmc cannot run inside the timed process),
+    for the matching grid cell, reorder interval and spin count; else None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        if ln.startswith("traffic_bytes_per_launch"):
+            f = dict(kv.split("=") for kv in ln.split()[1:])
+            if (abs(float(f["cell_um"]) - cell_um) < 1e-9 and int(f["sort"]) == sort_interval
+                    and int(f["spins"]) == n):
+                return int(f["bytes"])
+    return None
 
 
 def copy_probe(mib):
@@ -191,17 +223,18 @@ def main():
     n_tri, n_coll = None, None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(env, 20000, args.cpu_seconds, threads)
+        cpu = cpu_baseline(env, args.cpu_seconds, threads)
         n_tri, n_coll = cpu.pop("n_tri"), cpu.pop("n_coll")
         log(f"cpu baseline {cpu['value']:.3e} spin-steps/s ({cpu['sample']}); work model "
             f"n_tri {n_tri:.3f}, n_coll {n_coll:.4f}")
     roof = None
+    traffic = committed_traffic(cell_um, sort_interval, n)
     if n_tri is not None:
         F = flops_per_spin_step(n_tri, n_coll)
         avg_launch_s = step_ms / 1e3 / max(1, launches)
         achieved = F * n * steps_per_acq * args.steps / (step_ms / 1e3)
         roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
                 "flops_per_spin_step": F, "n_tri": n_tri, "n_coll": n_coll,
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "spin_steps_per_launch": n * steps_per_acq * args.steps / max(1, launches)}

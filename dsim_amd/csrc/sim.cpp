@@ -372,10 +372,6 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
             if (int rc = s->trace.alloc((size_t)iters * n * 5)) return rc;
         }
     }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, s->stream));
     uint32_t done = 0;
     const uint32_t chunk = s->sorting() ? std::min(s->prm.steps_per_launch, s->prm.sort_interval)
                                         : s->prm.steps_per_launch;
@@ -392,16 +388,22 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         P.step0 = s->step + done;
         P.inject = n3u ? s->inject.p + (size_t)done * n : nullptr;
         P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
+        // each step launch is bracketed by its own events: the timing covers
+        // the step kernel only, not the reorders between launches
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, s->stream));
         hipError_t e = dsim::launch_step(P, k, (int)s->prm.mode,
                                          s->prm.boundary == DSIM_BOUNDARY_PERIODIC, rng,
                                          trace != nullptr, moment, s->stream);
         if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
+        HIP_TRY(hipEventRecord(e1, s->stream));
+        s->ev.emplace_back(e0, e1);
         done += k;
         s->steps_since_sort += k;
         s->launches++;
     }
-    HIP_TRY(hipEventRecord(e1, s->stream));
-    s->ev.emplace_back(e0, e1);
     s->step += iters;
     if (trace) {
         HIP_TRY(hipMemcpyAsync(trace, s->trace.p, (size_t)iters * n * 20, hipMemcpyDeviceToHost,

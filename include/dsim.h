@@ -94,8 +94,9 @@ typedef struct {
     uint32_t rng;             /* DSIM_RNG_*; MWC is the reference's per-spin stream */
     uint64_t seed;            /* Philox key */
     float spin_radius;        /* clamp margin (spinSystem.cpp:61: 0.005) */
-    uint32_t sort_interval;   /* physical mode with Philox: reorder spins by cell every
-                               * this many steps (0 = never).  Results do not depend on it. */
+    uint32_t sort_interval;   /* physical mode with Philox: reorder spins in device memory
+                               * every this many steps (0 = never), by recent membrane-hit
+                               * rate, then cell.  Results do not depend on it. */
     uint32_t steps_per_launch; /* steps fused per kernel launch (0 = 256) */
 } dsim_params;
 
@@ -202,7 +203,8 @@ int dsim_get_moment(dsim_sim *sim, float *moment, float *x0);
 /* ---- timing ------------------------------------------------------------- */
 int dsim_sync(dsim_sim *sim);
 /* Milliseconds spent in step-kernel launches since the last reset, measured
- * with HIP events on the handle's stream; launches counts kernel launches. */
+ * with a HIP event pair around each launch on the handle's stream (spin
+ * reorders between launches are not included); launches counts them. */
 int dsim_step_timing(dsim_sim *sim, double *ms, uint64_t *launches);
 int dsim_step_timing_reset(dsim_sim *sim);
 
