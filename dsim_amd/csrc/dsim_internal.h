@@ -79,6 +79,21 @@ struct LabelParams {
 
 enum Rng { kRngInject = 0, kRngMwc = 1, kRngPhilox = 2 };
 
+// Job table of the batched walk (physical mode, Philox, moment accumulation):
+// job j advances the simulation of job[j].p by job[j].iters steps on blocks
+// [block_start[j], block_start[j+1]).
+constexpr uint32_t kMaxJobs = 256;
+struct WalkJob {
+    StepParams p;
+    uint32_t iters;
+};
+struct WalkTable {
+    uint32_t n_jobs;
+    uint32_t total_blocks;
+    uint32_t block_start[kMaxJobs + 1];
+    WalkJob job[kMaxJobs];
+};
+
 // Launchers (kernels.hip).  Return hipError_t of the launch.
 hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
                        bool trace, bool moment, hipStream_t s);
@@ -89,6 +104,10 @@ hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, 
                                 const float *grads, float pc, double *partial, uint32_t n_blocks,
                                 double *out, hipStream_t s);
 hipError_t launch_label(const LabelParams &p, hipStream_t s);
+// Batched physical-mode walk over a device-resident job table (periodic or
+// clamped boundary for every job of the launch).
+hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
+                             hipStream_t s);
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
                                uint32_t *label, uint32_t *id, float start_box, uint32_t key0,
                                uint32_t key1, hipStream_t s);

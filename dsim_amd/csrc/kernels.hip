@@ -760,14 +760,13 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return ((k / kXcdRun) * kXcds + x) * kXcdRun + k % kXcdRun;
 }
 
-// cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode, `iters` steps
-// per spin per launch.
+// cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode: `iters` steps
+// for the 256 spins of block `block` of the simulation described by P.
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
-__global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
-    __shared__ WaveScratch scratch[4];
-    WaveScratch &L = scratch[threadIdx.x >> 6];
+__device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
+                                          WaveScratch &L) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t i = block * blockDim.x + threadIdx.x;
     // lanes past the last spin stay in the loop (done) for the wave operations
     const bool live = i < P.n;
     uint32_t sid = 0, comp = 0, fiber = kFiberNone, s0 = 0, s1 = 0;
@@ -918,6 +917,42 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
         }
         if (P.hits) P.hits[i] += nhit;
     }
+}
+
+template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+__global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
+    __shared__ WaveScratch scratch[4];
+    walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(), iters, scratch[threadIdx.x >> 6]);
+}
+
+// Batched walk for the server: the blocks of one launch are shared out over
+// the jobs of a table in device memory (job j owns blocks [block_start[j],
+// block_start[j+1])).  Each job is a whole simulation (its own substrate,
+// spins and parameters) advanced by its own number of steps; its spins see
+// exactly the arithmetic of a k_walk launch of their own.  Blocks past the
+// table's total exit at once, so one launch shape (and one captured graph)
+// serves any mix of jobs.
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_walk_batch(const WalkTable *__restrict__ T) {
+    __shared__ WaveScratch scratch[4];
+    const uint32_t b = xcd_block();
+    if (b >= T->total_blocks) return;
+    uint32_t j = 0;
+    for (uint32_t st = kMaxJobs / 2; st; st >>= 1)
+        if (j + st < T->n_jobs && T->block_start[j + st] <= b) j += st;
+    const WalkJob &J = T->job[j];
+    walk_body<PERIODIC, kRngPhilox, false, true>(J.p, b - T->block_start[j], J.iters,
+                                                 scratch[threadIdx.x >> 6]);
+}
+
+hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
+                             hipStream_t s) {
+    if (grid_blocks == 0) return hipSuccess;
+    if (periodic)
+        hipLaunchKernelGGL(k_walk_batch<true>, dim3(grid_blocks), dim3(256), 0, s, table);
+    else
+        hipLaunchKernelGGL(k_walk_batch<false>, dim3(grid_blocks), dim3(256), 0, s, table);
+    return hipGetLastError();
 }
 
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>

@@ -13,6 +13,7 @@
 #include "../../include/dsim.h"
 #include "dsim_internal.h"
 #include "host_common.h"
+#include "sim_internal.h"
 
 using dsim_host::DevBuf;
 using dsim_host::fail;
@@ -610,4 +611,39 @@ extern "C" int dsim_step_timing_reset(dsim_sim *s) {
     s->ms_done = 0.0;
     s->launches = 0;
     return DSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Hooks for the batched server
+// ---------------------------------------------------------------------------
+
+int dsim_host::sim_batch_params(dsim_sim *s, float w, dsim::StepParams *P) {
+    if (int rc = check_sim(s)) return rc;
+    if (s->prm.mode != DSIM_MODE_PHYSICAL || s->prm.rng != DSIM_RNG_PHILOX || s->sorting())
+        return fail(DSIM_ERR_STATE, "batched steps need the physical mode, Philox, no reordering");
+    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "call dsim_moment_reset first");
+    *P = step_params(s);
+    P->weight = w;
+    P->step0 = s->step;  // the Philox counter continues where the handle stands
+    P->inject = nullptr;
+    P->trace = nullptr;
+    return DSIM_OK;
+}
+
+void dsim_host::sim_batch_advance(dsim_sim *s, float w, uint32_t iters) {
+    s->wsum += w * (float)iters;
+    s->step += iters;
+    s->launches++;
+}
+
+hipStream_t dsim_host::sim_stream(const dsim_sim *s) { return s->stream; }
+
+uint64_t dsim_host::sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_lines) {
+    const uint64_t m = std::max<uint32_t>(n, 1);
+    uint64_t b = m * F_COUNT * 4 + m * 4 + 16 + (uint64_t)dsim::kSignalBlocks * 15 * 8 + 15 * 8;
+    if (p->rng == DSIM_RNG_MWC) b += m * 8;
+    if (p->sort_interval > 0 && p->mode == DSIM_MODE_PHYSICAL && p->rng == DSIM_RNG_PHILOX)
+        b += m * F_COUNT * 4 + m * 16 + m * 8;  // second block, keys, values, sort scratch (about)
+    b += (uint64_t)n_lines * (12 + 256 * 15 * 8 + 15 * 8);  // readout
+    return b;
 }

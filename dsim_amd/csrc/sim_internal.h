@@ -1,0 +1,25 @@
+// Hooks of the simulation handle (sim.cpp) for the server (serve.cpp): the
+// server advances many handles with one batched step launch and then has to
+// do the bookkeeping a dsim_integrate_moment call would have done.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/dsim.h"
+#include "dsim_internal.h"
+
+namespace dsim_host {
+
+// Step parameters of `s` for one batched launch at gradient weight w.
+// Requires the physical mode, the Philox stream, no spin reordering and
+// accumulated moments (dsim_moment_reset called).
+int sim_batch_params(dsim_sim *s, float w, dsim::StepParams *P);
+// Records that a batched launch advanced `s` by `iters` steps at weight w.
+void sim_batch_advance(dsim_sim *s, float w, uint32_t iters);
+hipStream_t sim_stream(const dsim_sim *s);
+// Device bytes a handle of n spins holds, readout of n_lines lines included.
+uint64_t sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_lines);
+
+}  // namespace dsim_host
