@@ -98,6 +98,60 @@ def cpu_baseline(env, target_s, threads):
                 n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
 
 
+def served_bench(n_requests, plain_requests):
+    """The served form against the plain one on a stream of requests over
+    resident substrates: capacity (every request queued at once) in
+    simulations per hour, the same requests one at a time through the plain
+    handle, and Poisson streams at 40 % and 70 % of the measured capacity for
+    the latency percentiles (each request's latency also against its own
+    solo run)."""
+    from dsim_amd import served as SVD
+
+    sc = SVD.StreamConfig(n_requests=n_requests)
+    t0 = time.perf_counter()
+    envs = SVD.make_substrates(sc, log)
+    log(f"served: {sc.n_substrates} resident substrates in {time.perf_counter() - t0:.1f} s")
+    reqs = SVD.make_requests(sc, envs)
+    out, mk = SVD.run_served(envs, reqs, realtime=False)
+    cap = len(out) / mk
+    log(f"served: {len(out)} requests ({SVD.work(reqs):.3e} spin-steps) in {mk:.2f} s")
+    sub = reqs[:plain_requests]
+    plain = SVD.run_plain(envs, sub)
+    t_plain = sum(t for t, _ in plain)
+    same = all(np.array_equal(o["signals"], sig) for o, (_, sig) in zip(out, plain))
+    log(f"plain: {len(sub)} requests one at a time in {t_plain:.2f} s; served signals "
+        f"{'bit-identical' if same else 'DIFFER'}")
+    solo = np.array([t for t, _ in plain])
+    streams = []
+    for load in (0.4, 0.7):
+        rate = load * cap
+        live, _ = SVD.run_served(envs, SVD.make_requests(sc, envs, rate=rate), realtime=True)
+        lat = np.array([o["latency"] for o in live])
+        vs = lat[:len(solo)] / solo
+        streams.append({"load": load, "rate_per_s": rate,
+                        "p50_latency_s": float(np.percentile(lat, 50)),
+                        "p99_latency_s": float(np.percentile(lat, 99)),
+                        "p50_latency_vs_solo": float(np.percentile(vs, 50)),
+                        "p99_latency_vs_solo": float(np.percentile(vs, 99))})
+        log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity): latency p50 "
+            f"{np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s; vs solo p50 "
+            f"{np.percentile(vs, 50):.2f}x, p99 {np.percentile(vs, 99):.2f}x")
+    for e in envs:
+        e["sub"].close()
+    return {"requests": len(reqs), "substrates": sc.n_substrates,
+            "sims_per_hour": cap * 3600, "plain_sims_per_hour": len(sub) / t_plain * 3600,
+            "speedup_vs_plain": cap / (len(sub) / t_plain),
+            "served_spin_steps_per_s": SVD.work(reqs) / mk,
+            "streams": streams,
+            "bit_identical_to_plain": bool(same),
+            "workload": (f"{sc.n_substrates} substrates of {sc.axons[0]}-{sc.axons[1]} axons "
+                         f"(mean radius {sc.radius[0]}-{sc.radius[1]} um, {sc.ngon}-gon), "
+                         f"requests of {sc.spins[0]:.0e}-{sc.spins[1]:.0e} spins (log-uniform), "
+                         f"{sc.lines[0]}-{sc.lines[1]} lines, b {sc.b[0]}-{sc.b[1]} ms/um^2, "
+                         f"delta {sc.delta[0]}-{sc.delta[1]} ms, Delta {sc.Delta[0]}-{sc.Delta[1]} ms; "
+                         f"plain = first {len(sub)} requests one at a time")}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walk_scale-2000axon-1e6_hbm_traffic.md")
 
 
@@ -145,6 +199,9 @@ def main():
     ap.add_argument("--cell-um", type=float, default=0.0)
     ap.add_argument("--sort", type=int, default=-1)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=120,
+                    help="requests in the served-form comparison (0 skips it)")
+    ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -238,6 +295,11 @@ def main():
                 "flops_per_spin_step": F, "n_tri": n_tri, "n_coll": n_coll,
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "spin_steps_per_launch": n * steps_per_acq * args.steps / max(1, launches)}
+    served = None
+    if args.served_requests:
+        sim.close()
+        sub.close()
+        served = served_bench(args.served_requests, min(args.plain_requests, args.served_requests))
     out = {
         "metric": "spin-steps/s at 1M spins",
         "value": value,
@@ -262,6 +324,7 @@ def main():
         "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "served": served,
         "host": platform.processor() or platform.machine(),
     }
     print(json.dumps(out), flush=True)

@@ -33,6 +33,8 @@ using dsim_host::fail;
 
 namespace {
 
+// kActive covers admitted requests from preparation to readout; budget is
+// held from admission until the handle is released after the round.
 enum Status { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3 };
 
 struct Req {
@@ -45,6 +47,9 @@ struct Req {
     dsim_sim *sim = nullptr;
     uint32_t lobe = 0, lobe_left = 0;
     uint32_t chunk = 0;  // steps of the current round
+    bool ready = false;     // spins prepared: takes part in rounds
+    bool read_out = false;  // lobes done: signals formed at the next round
+    bool released = false;
     int status = kQueued;
     int err = DSIM_OK;
     std::string msg;
@@ -94,15 +99,7 @@ size_t table_bytes(uint32_t n_jobs) {
 }
 
 void finish(dsim_server *S, Req *r, int status, int err, const std::string &msg) {
-    if (r->sim) {
-        dsim_sim_destroy(r->sim);
-        r->sim = nullptr;
-    }
     std::lock_guard<std::mutex> l(S->mu);
-    if (r->status == kActive || r->status == kQueued) {
-        S->used -= r->bytes;
-        S->active_spins -= r->r.n_spins;
-    }
     r->status = status;
     r->err = err;
     r->msg = msg;
@@ -139,6 +136,24 @@ int start_request(dsim_server *S, Req *r) {
     return DSIM_OK;
 }
 
+// Frees a finished request's handle and returns its budget (called with the
+// device idle for the server stream, so the frees do not stall a round).
+void release(dsim_server *S, Req *r) {
+    if (r->released) return;
+    if (r->sim) {
+        dsim_sim_destroy(r->sim);
+        r->sim = nullptr;
+    }
+    std::lock_guard<std::mutex> l(S->mu);
+    S->used -= r->bytes;
+    S->active_spins -= r->r.n_spins;
+    r->released = true;
+}
+
+// Workgroups of the step kernel resident at once on the device (256 CUs, four
+// 256-spin workgroups each at its register and LDS use).
+constexpr uint32_t kResidentBlocks = 1024;
+
 int ensure_batch(dsim_server *S, int kind) {
     Batch &B = S->batch[kind];
     if (B.exec) return DSIM_OK;
@@ -159,10 +174,69 @@ int ensure_batch(dsim_server *S, int kind) {
     return DSIM_OK;
 }
 
+// One round: (1) launch the batched step for every prepared request; (2)
+// while it runs, form the signals of requests whose lobes ended last round
+// and prepare newly admitted requests (their handles' own streams); (3) wait
+// for the step; (4) advance the bookkeeping; (5) release finished requests.
 int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
     std::lock_guard<std::mutex> rl(S->round_mu);
-    // admission: first come, first served, while budget, spin limit and job
-    // table room allow
+    // (1) one batched launch per boundary kind carries every prepared request.
+    // A round lasts about total_blocks / kResidentBlocks times as long as one
+    // block's round_steps steps, so a job of few blocks can walk that many
+    // more steps in it without making it longer: small jobs get longer chunks
+    // (fewer rounds, lower latency) and go first in the table, so their
+    // blocks are dispatched first.
+    bool launched = false;
+    for (int kind = 0; kind < 2; ++kind) {
+        std::vector<Req *> jobs;
+        uint32_t total = 0;
+        for (Req *r : S->active) {
+            if (!r->ready || r->read_out || r->r.boundary != (uint32_t)kind || r->lobe_left == 0)
+                continue;
+            jobs.push_back(r);
+            total += (r->r.n_spins + 255) / 256;
+        }
+        std::stable_sort(jobs.begin(), jobs.end(),
+                         [](const Req *a, const Req *b) { return a->r.n_spins < b->r.n_spins; });
+        const uint32_t stretch = std::max<uint32_t>(1, total / kResidentBlocks);
+        uint32_t nj = 0, blocks = 0;
+        dsim::WalkTable *T = nullptr;
+        for (Req *r : jobs) {
+            if (!T) {
+                if (int rc = ensure_batch(S, kind)) return rc;
+                T = S->batch[kind].host;
+            }
+            const uint32_t nb = (r->r.n_spins + 255) / 256;
+            const uint32_t want = nb * 8 <= total ? S->round_steps * stretch : S->round_steps;
+            r->chunk = std::min(want, r->lobe_left);
+            dsim::WalkJob &J = T->job[nj];
+            if (int rc = dsim_host::sim_batch_params(r->sim, r->lobe_w[r->lobe], &J.p)) return rc;
+            J.iters = r->chunk;
+            T->block_start[nj] = blocks;
+            blocks += (r->r.n_spins + 255) / 256;
+            nj++;
+        }
+        if (!nj) continue;
+        T->block_start[nj] = blocks;
+        T->n_jobs = nj;
+        T->total_blocks = blocks;
+        HIP_TRY(hipMemcpyAsync(S->batch[kind].dev, T, table_bytes(nj), hipMemcpyHostToDevice,
+                               S->stream));
+        HIP_TRY(hipGraphLaunch(S->batch[kind].exec, S->stream));
+        launched = true;
+    }
+    // (2a) signals of the requests whose lobes ended last round
+    for (Req *r : S->active) {
+        if (!r->read_out || r->status != kActive) continue;
+        r->signals.assign((size_t)r->r.n_lines * 5, 0.0);
+        int rc = r->r.n_lines ? dsim_signal_moment(r->sim, r->r.n_lines, r->grads.data(),
+                                                   r->r.phase_constant, r->signals.data())
+                              : dsim_sync(r->sim);
+        if (rc) finish(S, r, kFailed, rc, dsim_last_error());
+        else finish(S, r, kDone, DSIM_OK, "");
+    }
+    // (2b) admission: first come, first served, while budget, spin limit and
+    // job table room allow; spins prepared while the step runs
     std::vector<Req *> admitted;
     {
         std::lock_guard<std::mutex> l(S->mu);
@@ -183,42 +257,13 @@ int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
     }
     for (Req *r : admitted) {
         r->st.admit_s = S->clock();
-        if (int rc = start_request(S, r)) {
-            finish(S, r, kFailed, rc, dsim_last_error());
-            continue;
-        }
         S->active.push_back(r);
+        if (int rc = start_request(S, r)) finish(S, r, kFailed, rc, dsim_last_error());
+        else r->ready = true;
     }
-    // one batched launch per boundary kind carries every active request
-    bool launched = false;
-    for (int kind = 0; kind < 2; ++kind) {
-        uint32_t nj = 0, blocks = 0;
-        dsim::WalkTable *T = nullptr;
-        for (Req *r : S->active) {
-            if (r->r.boundary != (uint32_t)kind || r->lobe_left == 0) continue;
-            if (!T) {
-                if (int rc = ensure_batch(S, kind)) return rc;
-                T = S->batch[kind].host;
-            }
-            r->chunk = std::min(S->round_steps, r->lobe_left);
-            dsim::WalkJob &J = T->job[nj];
-            if (int rc = dsim_host::sim_batch_params(r->sim, r->lobe_w[r->lobe], &J.p)) return rc;
-            J.iters = r->chunk;
-            T->block_start[nj] = blocks;
-            blocks += (r->r.n_spins + 255) / 256;
-            nj++;
-        }
-        if (!nj) continue;
-        T->block_start[nj] = blocks;
-        T->n_jobs = nj;
-        T->total_blocks = blocks;
-        HIP_TRY(hipMemcpyAsync(S->batch[kind].dev, T, table_bytes(nj), hipMemcpyHostToDevice,
-                               S->stream));
-        HIP_TRY(hipGraphLaunch(S->batch[kind].exec, S->stream));
-        launched = true;
-    }
+    // (3) the step
     if (launched) HIP_TRY(hipStreamSynchronize(S->stream));
-    // bookkeeping; read out and release the requests whose lobes are done
+    // (4) bookkeeping; (5) release what is finished
     std::vector<Req *> still;
     for (Req *r : S->active) {
         if (r->chunk) {
@@ -230,16 +275,9 @@ int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
             while (r->lobe_left == 0 && r->lobe + 1 < r->lobe_n.size())
                 r->lobe_left = r->lobe_n[++r->lobe];
         }
-        if (r->lobe_left) {
-            still.push_back(r);
-            continue;
-        }
-        r->signals.assign((size_t)r->r.n_lines * 5, 0.0);
-        int rc = r->r.n_lines ? dsim_signal_moment(r->sim, r->r.n_lines, r->grads.data(),
-                                                   r->r.phase_constant, r->signals.data())
-                              : dsim_sync(r->sim);
-        if (rc) finish(S, r, kFailed, rc, dsim_last_error());
-        else finish(S, r, kDone, DSIM_OK, "");
+        if (r->ready && r->lobe_left == 0) r->read_out = true;
+        if (r->status == kDone || r->status == kFailed) release(S, r);
+        else still.push_back(r);
     }
     S->active.swap(still);
     std::lock_guard<std::mutex> l(S->mu);
@@ -261,7 +299,11 @@ void loop_main(dsim_server *S) {
             // a failed launch takes every active request with it
             std::string msg = dsim_last_error();
             std::lock_guard<std::mutex> rl(S->round_mu);
-            for (Req *r : S->active) finish(S, r, kFailed, rc, msg);
+            (void)hipStreamSynchronize(S->stream);
+            for (Req *r : S->active) {
+                finish(S, r, kFailed, rc, msg);
+                release(S, r);
+            }
             S->active.clear();
             std::lock_guard<std::mutex> l(S->mu);
             S->n_active = 0;
