@@ -33,7 +33,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFOR
              "-Wall", "-Wno-misleading-indentation", "-Wno-unused-result", "-pthread"]
 
 HIP_SOURCES = ["kernels.hip"]
-CXX_SOURCES = ["host.cpp", "sim.cpp", "formats.cpp", "serve.cpp"]
+CXX_SOURCES = ["host.cpp", "sim.cpp", "formats.cpp", "serve.cpp", "phase.cpp"]
 MAIN_SOURCES = ["spin_system.cpp", "dsim_main.cpp"]
 
 

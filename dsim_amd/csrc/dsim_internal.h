@@ -52,6 +52,9 @@ struct StepParams {
     float *momc;      // [3][n] Kahan compensation
     uint32_t *hits;   // membrane hits per spin since the last reorder, or null
     uint32_t hit_window;  // steps those counts cover (0: no history)
+    float *traj;          // [3*iters][traj_stride] displacement after each step of the
+                          // launch, by spin id (moment mode), or null
+    uint32_t traj_stride;
     // per call
     float gx, gy, gz, pc, weight;
     uint32_t key0, key1;
@@ -116,6 +119,26 @@ hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const
                                hipStream_t s);
 
 constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout kernels
+
+// Phase of arbitrary gradient waveforms (see kernels.hip, k_phase_gemm):
+// phi[s][w] += sum_k traj[k][s] * G[w][k] over one chunk of K = 3*steps,
+// with G given as bf16 hi and lo parts [n_wave][k_total] (k contiguous) and
+// phi kept Kahan-compensated in comp.  Spins padded to kPhaseTileM in traj.
+constexpr uint32_t kPhaseTileM = 128;  // spins per workgroup tile
+constexpr uint32_t kPhaseTileN = 256;  // waveforms per workgroup tile
+constexpr uint32_t kPhaseSliceK = 32;  // k per LDS stage
+hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
+                             const uint16_t *g_hi, const uint16_t *g_lo, uint32_t k_total,
+                             uint32_t k0, uint32_t n_wave, float *phi, float *comp,
+                             hipStream_t s);
+// Signal of every waveform: per spin block and waveform the FP64 sums of
+// m cos(phase), m sin(phase) and counts for the total and compartments 0-3,
+// phase = phi[s][w] + gsum[w] . x0[s] (in FP64); then a fixed-order final
+// pass into out[n_wave][15].
+hipError_t launch_phase_signal(uint32_t n, const float *phi, uint32_t n_wave,
+                               const double *gsum, const float *mag, const uint32_t *label,
+                               const float *x0, double *partial, uint32_t n_blocks,
+                               double *out, hipStream_t s);
 
 // Spatial reordering of spins: Morton key of each slot's cell, radix sort of
 // (key, slot), then a gather of every per-spin field into its new slot.

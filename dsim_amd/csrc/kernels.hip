@@ -893,6 +893,13 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
                 yv = fmaf(P.weight, dy, -cy); tv = my + yv; cy = (tv - my) - yv; my = tv;
                 yv = fmaf(P.weight, dz, -cz); tv = mz + yv; cz = (tv - mz) - yv; mz = tv;
             }
+            if (MOMENT && P.traj) {
+                // unwrapped displacement after this step, rows 3*it + axis
+                float *tr = P.traj + (size_t)(3 * it) * P.traj_stride + sid;
+                tr[0] = dx;
+                tr[P.traj_stride] = dy;
+                tr[2 * (size_t)P.traj_stride] = dz;
+            }
             if (TRACE) {
                 uint32_t qx, qy, qz;
                 cell_of<1>(P, x, y, z, qx, qy, qz);
@@ -993,6 +1000,199 @@ hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int period
     if (mode == 0) return launch2<0, false>(p, iters, rng, trace, moment, s);
     if (periodic) return launch2<1, true>(p, iters, rng, trace, moment, s);
     return launch2<1, false>(p, iters, rng, trace, moment, s);
+}
+
+// ------------------------------------------------------------------------ //
+// Phase of arbitrary gradient waveforms on the matrix cores.
+//
+// Over a chunk of T steps the phase increment of spin s under waveform w is
+// sum_t pc G_w(t) . u_s(t) with u the unwrapped displacement: a GEMM of the
+// trajectory (spins x K, K = 3T) by the waveforms (K x n_wave).  Both factors
+// are split into bf16 hi + lo parts (x = hi + lo to about 16 bits) and the
+// product is hi*hi + hi*lo + lo*hi, accumulated in FP32 by
+// v_mfma_f32_32x32x16_bf16; the chunk's result is folded into the phase with
+// Kahan compensation.
+//
+// Workgroup: 256 threads, tile of 128 spins x 256 waveforms, four waves of
+// 64 x 128 (2 x 4 MFMA tiles of 32 x 32).  K advances in slices of 32: the
+// trajectory slice (fp32, spins contiguous) is split and transposed into LDS
+// as [spin][k], the waveform slice is copied as [waveform][k], so every MFMA
+// operand (8 consecutive k of one row) is one 16-byte LDS read.
+// v_mfma_f32_32x32x16_bf16 lane layout (checked on the device): A row l%32,
+// B column l%32, k = 8*(l/32) + v for v < 8; result register r holds row
+// 8*(r/4) + 4*(l/32) + r%4, column l%32.
+// ------------------------------------------------------------------------ //
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kLdsPitch = 40;  // bf16 per LDS row: 32 k + 8 padding (80 bytes)
+
+__device__ __forceinline__ uint16_t bf16_bits_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ traj, uint32_t n,
+                                                    uint32_t n_pad, uint32_t k_chunk,
+                                                    const uint16_t *__restrict__ g_hi,
+                                                    const uint16_t *__restrict__ g_lo,
+                                                    uint32_t k_total, uint32_t k0,
+                                                    uint32_t n_wave, float *__restrict__ phi,
+                                                    float *__restrict__ comp) {
+    __shared__ __align__(16) uint16_t a_hi[kPhaseTileM * kLdsPitch];
+    __shared__ __align__(16) uint16_t a_lo[kPhaseTileM * kLdsPitch];
+    __shared__ __align__(16) uint16_t b_hi[kPhaseTileN * kLdsPitch];
+    __shared__ __align__(16) uint16_t b_lo[kPhaseTileN * kLdsPitch];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t n0 = blockIdx.x * kPhaseTileN;  // waveform tile
+    const uint32_t m0 = blockIdx.y * kPhaseTileM;  // spin tile
+    const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 128u;  // wave tile in the WG tile
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    for (uint32_t ks = 0; ks < k_chunk; ks += kPhaseSliceK) {
+        __syncthreads();
+        // trajectory slice: 32 rows x 128 spins (fp32) -> hi/lo [spin][k]
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t f = t + 256u * q;  // float4 index in the slice
+            const uint32_t kr = f >> 5, m4 = (f & 31u) * 4u;
+            const float4 v =
+                *reinterpret_cast<const float4 *>(traj + (size_t)(ks + kr) * n_pad + m0 + m4);
+            const float xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint16_t h = bf16_bits_rne(xs[e]);
+                a_hi[(m4 + e) * kLdsPitch + kr] = h;
+                a_lo[(m4 + e) * kLdsPitch + kr] = bf16_bits_rne(xs[e] - bf16_to_float(h));
+            }
+        }
+        // waveform slice: 256 rows x 32 k (bf16, k contiguous), 64 bytes a row
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t c = t + 256u * q;  // 16-byte piece: row c/4, quarter c%4
+            const uint32_t row = c >> 2, part = (c & 3u) * 8u;
+            const size_t src = (size_t)(n0 + row) * k_total + k0 + ks + part;
+            *reinterpret_cast<uint4 *>(&b_hi[row * kLdsPitch + part]) =
+                *reinterpret_cast<const uint4 *>(g_hi + src);
+            *reinterpret_cast<uint4 *>(&b_lo[row * kLdsPitch + part]) =
+                *reinterpret_cast<const uint4 *>(g_lo + src);
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
+            const uint32_t kofs = kk + 8u * (lane >> 5);
+            bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint32_t row = wm + 32u * i + (lane & 31u);
+                ah[i] = *reinterpret_cast<const bf16x8 *>(&a_hi[row * kLdsPitch + kofs]);
+                al[i] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * kLdsPitch + kofs]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t row = wn + 32u * j + (lane & 31u);
+                bh[j] = *reinterpret_cast<const bf16x8 *>(&b_hi[row * kLdsPitch + kofs]);
+                bl[j] = *reinterpret_cast<const bf16x8 *>(&b_lo[row * kLdsPitch + kofs]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    // Kahan fold into phi / comp ([spin][waveform]); lanes 0-31 and 32-63
+    // each store 32 consecutive waveforms of one spin
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t s = m0 + wm + 32u * i + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
+                const uint32_t w = n0 + wn + 32u * j + (lane & 31u);
+                if (s >= n) continue;
+                const size_t o = (size_t)s * n_wave + w;
+                const float p = phi[o], c = comp[o];
+                const float y = acc[i][j][r] - c;
+                const float tt = p + y;
+                comp[o] = (tt - p) - y;
+                phi[o] = tt;
+            }
+}
+
+hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
+                             const uint16_t *g_hi, const uint16_t *g_lo, uint32_t k_total,
+                             uint32_t k0, uint32_t n_wave, float *phi, float *comp,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    dim3 grid(n_wave / kPhaseTileN, n_pad / kPhaseTileM);
+    hipLaunchKernelGGL(k_phase_gemm, grid, dim3(256), 0, s, traj, n, n_pad, k_chunk, g_hi, g_lo,
+                       k_total, k0, n_wave, phi, comp);
+    return hipGetLastError();
+}
+
+// Readout of the waveform phases: one thread per waveform, one block row per
+// range of spins; FP64 phase (x0 term added in FP64), cos/sin and sums.
+__global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *__restrict__ phi,
+                                                       uint32_t n_wave,
+                                                       const double *__restrict__ gsum,
+                                                       const float *__restrict__ mag,
+                                                       const uint32_t *__restrict__ label,
+                                                       const float *__restrict__ x0,
+                                                       uint32_t per_block, double *partial) {
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t sb = blockIdx.y;
+    if (w >= n_wave) return;
+    const double gx = gsum[3 * w], gy = gsum[3 * w + 1], gz = gsum[3 * w + 2];
+    double v[15];
+    for (int k = 0; k < 15; ++k) v[k] = 0.0;
+    const uint32_t s1 = min(n, (sb + 1) * per_block);
+    for (uint32_t s = sb * per_block; s < s1; ++s) {
+        const double ph = (double)phi[(size_t)s * n_wave + w] + gx * (double)x0[s] +
+                          gy * (double)x0[n + s] + gz * (double)x0[2 * n + s];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        const double m = (double)mag[s];
+        const uint32_t c = (label[s] & 0xFFu) + 1u;
+        v[0] += m * cs; v[5] += m * sn; v[10] += 1.0;
+        if (c < 5) { v[c] += m * cs; v[5 + c] += m * sn; v[10 + c] += 1.0; }
+    }
+    double *o = partial + ((size_t)sb * n_wave + w) * 15;
+    for (int k = 0; k < 15; ++k) o[k] = v[k];
+}
+
+__global__ __launch_bounds__(256) void k_phase_final(const double *partial, uint32_t n_blocks,
+                                                     uint32_t n_wave, double *out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // waveform*15 + quantity
+    if (i >= n_wave * 15u) return;
+    double acc = 0.0;
+    for (uint32_t b = 0; b < n_blocks; ++b) acc += partial[(size_t)b * n_wave * 15 + i];
+    out[i] = acc;
+}
+
+hipError_t launch_phase_signal(uint32_t n, const float *phi, uint32_t n_wave,
+                               const double *gsum, const float *mag, const uint32_t *label,
+                               const float *x0, double *partial, uint32_t n_blocks,
+                               double *out, hipStream_t s) {
+    if (n_wave == 0 || n_blocks == 0) return hipSuccess;
+    const uint32_t per = (n + n_blocks - 1) / n_blocks;
+    hipLaunchKernelGGL(k_phase_partial, dim3((n_wave + 255) / 256, n_blocks), dim3(256), 0, s, n,
+                       phi, n_wave, gsum, mag, label, x0, per, partial);
+    hipLaunchKernelGGL(k_phase_final, dim3((n_wave * 15 + 255) / 256), dim3(256), 0, s, partial,
+                       n_blocks, n_wave, out);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------ //

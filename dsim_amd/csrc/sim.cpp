@@ -647,3 +647,29 @@ uint64_t dsim_host::sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_l
     b += (uint64_t)n_lines * (12 + 256 * 15 * 8 + 15 * 8);  // readout
     return b;
 }
+
+int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj, uint32_t traj_stride) {
+    if (int rc = check_sim(s)) return rc;
+    if (s->prm.mode != DSIM_MODE_PHYSICAL || s->sorting())
+        return fail(DSIM_ERR_STATE, "waveform runs need the physical mode without reordering");
+    if (!s->moment_ready) return fail(DSIM_ERR_STATE, "call dsim_moment_reset first");
+    dsim::StepParams P = step_params(s);
+    P.weight = 0.0f;
+    P.traj = traj;
+    P.traj_stride = traj_stride;
+    // one launch for the whole chunk: the kernel writes rows 3*it..3*it+2
+    const uint32_t spl = s->prm.steps_per_launch;
+    s->prm.steps_per_launch = iters;
+    int rc = run_steps(s, P, iters, rng_kind(s, nullptr), true, nullptr, nullptr);
+    s->prm.steps_per_launch = spl;
+    return rc;
+}
+
+void dsim_host::sim_readout_fields(dsim_sim *s, const float **mag, const uint32_t **label,
+                                   const float **x0) {
+    *mag = s->ff(F_MAG);
+    *label = s->f(F_LABEL);
+    *x0 = s->ff(F_X0);
+}
+
+uint32_t dsim_host::sim_spins(const dsim_sim *s) { return s->n; }

@@ -19,6 +19,14 @@ int sim_batch_params(dsim_sim *s, float w, dsim::StepParams *P);
 // Records that a batched launch advanced `s` by `iters` steps at weight w.
 void sim_batch_advance(dsim_sim *s, float w, uint32_t iters);
 hipStream_t sim_stream(const dsim_sim *s);
+// Advances `s` by `iters` steps (moment mode, weight 0: no moment change)
+// writing the unwrapped displacement after every step to traj
+// [3*iters][traj_stride] by spin id.  Requires dsim_moment_reset and no
+// reordering (the phase readout pairs traj rows with state slots).
+int sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj, uint32_t traj_stride);
+// Device pointers of the state fields the waveform readout needs.
+void sim_readout_fields(dsim_sim *s, const float **mag, const uint32_t **label, const float **x0);
+uint32_t sim_spins(const dsim_sim *s);
 // Device bytes a handle of n spins holds, readout of n_lines lines included.
 uint64_t sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_lines);
 

@@ -1,0 +1,59 @@
+/*
+ * dsim waveforms: the phase of many arbitrary gradient waveforms from one walk
+ * (C ABI).
+ *
+ * No single counterpart in the reference: its phase update takes one
+ * gradient vector per integrate call (spinKernelCpu.cpp:490, spinKernel.cu:811)
+ * and its sequencer plays only piecewise PGSE lines (diffusion.cpp:564-585),
+ * re-walking the spins for each.  Here one walk of a simulation handle serves
+ * every waveform: the unwrapped displacement of every spin is recorded chunk by
+ * chunk and multiplied on the matrix cores by the waveforms' gradients, the
+ * phase of each (spin, waveform) pair kept in FP32 with Kahan compensation
+ * across chunks (hi/lo bf16 split products, FP32 accumulation), and the signals
+ * are summed in FP64.
+ *
+ * The handle must use the physical mode without spin reordering
+ * (sort_interval 0), and dsim_moment_reset must have been called: the phase is
+ * pc * sum_t G_w(t) . (x0 + u(t)) with x0 the positions at that reset and u the
+ * unwrapped displacement since.
+ */
+#ifndef DSIM_PHASE_H
+#define DSIM_PHASE_H
+
+#include <stdint.h>
+
+#include "dsim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dsim_waveforms dsim_waveforms;
+
+/* n_wave: a multiple of 256; n_steps: a multiple of chunk_steps; chunk_steps:
+ * a multiple of 32 (64 gives the GEMM K = 192); grads: host array
+ * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
+ * phase_constant = 2*pi*gamma*dt.  Device memory: 8 bytes per (spin,
+ * waveform) for the phase and its compensation, plus 4 * n_wave * 3 * n_steps
+ * bytes of bf16 gradients and one chunk of trajectory. */
+int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_steps, const float *grads,
+                          float phase_constant, uint32_t chunk_steps, dsim_waveforms **out);
+int dsim_waveforms_destroy(dsim_waveforms *wf);
+/* Walks `steps` more steps (a multiple of chunk_steps; the total may not pass
+ * n_steps) and accumulates the phases.  traj_out (may be NULL): host array
+ * [steps][3][n_spins] that receives the recorded displacements (for checks). */
+int dsim_waveforms_run(dsim_waveforms *wf, uint32_t steps, float *traj_out);
+/* Signals per waveform, [n_wave*5]: total, then compartments 0-3, as
+ * getMrSignal (spinSystem.cpp:1452-1500), from the FP64 sums. */
+int dsim_waveforms_signal(dsim_waveforms *wf, double *out);
+/* Host copy of the accumulated phases without the x0 term, [n_spins][n_wave],
+ * and their Kahan compensations (either may be NULL). */
+int dsim_waveforms_get_phase(dsim_waveforms *wf, float *phase, float *comp);
+/* Milliseconds spent in the walk and in the phase GEMM since creation. */
+int dsim_waveforms_timing(dsim_waveforms *wf, double *walk_ms, double *gemm_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
