@@ -152,6 +152,92 @@ def served_bench(n_requests, plain_requests):
                          f"plain = first {len(sub)} requests one at a time")}
 
 
+PEAK_BF16 = 2.5e15      # MI355X dense BF16 matrix, FLOP/s
+
+
+def waveform_bench(n_spins, n_wave, sample):
+    """Waveform-phase path: n_spins spins on the small substrate, n_wave random
+    piecewise-constant waveforms (|G| <= 80 mT/m, a new vector every 10 us) over
+    12,000 steps in chunks of 64; the phase GEMM against the BF16 matrix peak;
+    per-waveform error against FP64 on a `sample`-spin run of the same
+    substrate and waveforms."""
+    from dsim_amd import phase as PH
+
+    cfg = S.SMALL
+    steps, chunk = 12000, 64
+    t0 = time.perf_counter()
+    env = S.build(cfg, cell_um=0.2, axons=S.pack(cfg))
+    g = env["grid"]
+    sub = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
+                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+    pc = np.float32(2 * np.pi * GAMMA * cfg.dt)
+    grads = PH.random_waveforms(n_wave, steps, seed=17) * np.float32(1e-9 * env["space_scale"])
+    log(f"waveforms: substrate and {n_wave} waveforms in {time.perf_counter() - t0:.1f} s")
+
+    def prepared(n, seed):
+        sim = D.Simulation(sub, n, dt=cfg.dt, std_dev=env["std_dev"], t2=env["t2"],
+                           permeability=env["permeability"], mode=D.MODE_PHYSICAL,
+                           boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=seed)
+        sim.init_uniform(1.0)
+        sim.label(env["xmax"], env["xmax"], env["xmax"])
+        sim.moment_reset()
+        return sim
+
+    sim = prepared(n_spins, 3)
+    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk)
+    t0 = time.perf_counter()
+    wf.run(steps)
+    sig = wf.signal()
+    wall = time.perf_counter() - t0
+    walk_ms, gemm_ms = wf.timing()
+    wf.close()
+    sim.close()
+    flops = 3 * 2.0 * n_spins * n_wave * 3 * steps   # hi*hi + hi*lo + lo*hi
+    log(f"waveforms: {n_spins} spins x {steps} steps x {n_wave} waveforms in {wall:.2f} s "
+        f"(walk {walk_ms / 1e3:.2f} s, phase GEMM {gemm_ms / 1e3:.2f} s = "
+        f"{flops / (gemm_ms / 1e3) / 1e12:.0f} TFLOP/s); b=0-free signal range "
+        f"{sig[:, 0].min():.4f}-{sig[:, 0].max():.4f}")
+    # FP64 check on a sample: the same product against a float64 GEMM of the
+    # recorded trajectory (numpy, checker only)
+    sim = prepared(sample, 4)
+    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk)
+    traj = wf.run(steps, trajectory=True)
+    phi, _ = wf.phase()
+    got = wf.signal()
+    st = sim.get_state()
+    _, x0 = sim.get_moment()
+    wf.close()
+    sim.close()
+    t0 = time.perf_counter()
+    coef = (pc * grads).astype(np.float32).astype(np.float64)          # [steps, n_wave, 3]
+    A = traj.transpose(2, 0, 1).reshape(sample, -1).astype(np.float64)
+    del traj
+    B = np.ascontiguousarray(coef.transpose(0, 2, 1).reshape(-1, n_wave))
+    ref = A @ B
+    del A
+    dphi = np.abs(phi.astype(np.float64) - ref).max(0)
+    tot = ref + x0.astype(np.float64) @ coef.sum(0).T
+    z = st["mag"].astype(np.float64)[:, None] * np.exp(1j * tot)
+    want = np.abs(z.sum(0)) / sample
+    dsig = np.abs(got[:, 0] - want)
+    log(f"waveforms: FP64 reference in {time.perf_counter() - t0:.1f} s")
+    log(f"waveforms vs FP64 ({sample} spins): phase error per waveform max {dphi.max():.2e} rad "
+        f"(median {np.median(dphi):.2e}); signal error max {dsig.max():.2e} (median "
+        f"{np.median(dsig):.2e})")
+    sub.close()
+    return {"spins": n_spins, "waveforms": n_wave, "steps": steps, "chunk_steps": chunk,
+            "gemm_k": 3 * chunk, "substrate": cfg.name, "seconds": wall,
+            "spin_steps_per_s": n_spins * steps / wall, "walk_s": walk_ms / 1e3,
+            "phase_gemm_s": gemm_ms / 1e3,
+            "phase_gemm_roofline": {"bound": "mfma", "achieved": flops / (gemm_ms / 1e3) / 1e12,
+                                    "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                                    "frac": flops / (gemm_ms / 1e3) / PEAK_BF16},
+            "fp64_check": {"spins": sample, "phase_err_max_rad": float(dphi.max()),
+                           "phase_err_median_rad": float(np.median(dphi)),
+                           "signal_err_max": float(dsig.max()),
+                           "signal_err_median": float(np.median(dsig))}}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walk_scale-2000axon-1e6_hbm_traffic.md")
 
 
@@ -202,6 +288,9 @@ def main():
     ap.add_argument("--served-requests", type=int, default=120,
                     help="requests in the served-form comparison (0 skips it)")
     ap.add_argument("--plain-requests", type=int, default=40)
+    ap.add_argument("--waveform-spins", type=int, default=4_000_000,
+                    help="spins of the waveform-phase measurement (0 skips it)")
+    ap.add_argument("--waveforms", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -300,6 +389,9 @@ def main():
         sim.close()
         sub.close()
         served = served_bench(args.served_requests, min(args.plain_requests, args.served_requests))
+    waves = None
+    if args.waveform_spins:
+        waves = waveform_bench(args.waveform_spins, args.waveforms, 8192)
     out = {
         "metric": "spin-steps/s at 1M spins",
         "value": value,
@@ -325,6 +417,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "served": served,
+        "waveforms": waves,
         "host": platform.processor() or platform.machine(),
     }
     print(json.dumps(out), flush=True)

@@ -1013,8 +1013,8 @@ hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int period
 // v_mfma_f32_32x32x16_bf16; the chunk's result is folded into the phase with
 // Kahan compensation.
 //
-// Workgroup: 256 threads, tile of 128 spins x 256 waveforms, four waves of
-// 64 x 128 (2 x 4 MFMA tiles of 32 x 32).  K advances in slices of 32: the
+// Workgroup: 256 threads, tile of 128 spins x 128 waveforms, four waves of
+// 64 x 64 (2 x 2 MFMA tiles of 32 x 32), several workgroups per CU.  K advances in slices of 32: the
 // trajectory slice (fp32, spins contiguous) is split and transposed into LDS
 // as [spin][k], the waveform slice is copied as [waveform][k], so every MFMA
 // operand (8 consecutive k of one row) is one 16-byte LDS read.
@@ -1039,44 +1039,64 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
                                                     const uint16_t *__restrict__ g_hi,
                                                     const uint16_t *__restrict__ g_lo,
                                                     uint32_t k_total, uint32_t k0,
-                                                    uint32_t n_wave, float *__restrict__ phi,
-                                                    float *__restrict__ comp) {
+                                                    uint32_t n_wave, float2 *__restrict__ phi) {
     __shared__ __align__(16) uint16_t a_hi[kPhaseTileM * kLdsPitch];
     __shared__ __align__(16) uint16_t a_lo[kPhaseTileM * kLdsPitch];
     __shared__ __align__(16) uint16_t b_hi[kPhaseTileN * kLdsPitch];
     __shared__ __align__(16) uint16_t b_lo[kPhaseTileN * kLdsPitch];
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const uint32_t n0 = blockIdx.x * kPhaseTileN;  // waveform tile
-    const uint32_t m0 = blockIdx.y * kPhaseTileM;  // spin tile
-    const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 128u;  // wave tile in the WG tile
-    f32x16 acc[2][4];
+    // Workgroups go to the 8 XCDs in turn; the numbering below hands all
+    // waveform tiles of one spin tile to one XCD back to back, so the
+    // trajectory slice is read from HBM once and then from that XCD's L2.
+    const uint32_t n_tiles = n_wave / kPhaseTileN, m_tiles = n_pad / kPhaseTileM;
+    const uint32_t xcd = blockIdx.x % 8u, r = blockIdx.x / 8u;
+    const uint32_t tm = (r / n_tiles) * 8u + xcd;
+    if (tm >= m_tiles) return;
+    const uint32_t n0 = (r % n_tiles) * kPhaseTileN;  // waveform tile
+    const uint32_t m0 = tm * kPhaseTileM;             // spin tile
+    const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 64u;  // wave tile in the WG tile
+    f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
     for (uint32_t ks = 0; ks < k_chunk; ks += kPhaseSliceK) {
         __syncthreads();
-        // trajectory slice: 32 rows x 128 spins (fp32) -> hi/lo [spin][k]
+        // trajectory slice: 32 rows x 128 spins (fp32) -> hi/lo [spin][k].
+        // Thread: one spin, 16 consecutive k (loads coalesced over the spins);
+        // its 16 hi and 16 lo values go to LDS as two 16-byte rows each, which
+        // at an 80-byte pitch land on distinct banks for 8 consecutive spins.
+        {
+            const uint32_t m = t & 127u, kg = (t >> 7) * 16u;
+            const float *src = traj + (size_t)(ks + kg) * n_pad + m0 + m;
+            uint16_t h[16], l[16];
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t f = t + 256u * q;  // float4 index in the slice
-            const uint32_t kr = f >> 5, m4 = (f & 31u) * 4u;
-            const float4 v =
-                *reinterpret_cast<const float4 *>(traj + (size_t)(ks + kr) * n_pad + m0 + m4);
-            const float xs[4] = {v.x, v.y, v.z, v.w};
+            for (uint32_t q = 0; q < 16; ++q) {
+                const float x = src[(size_t)q * n_pad];
+                h[q] = bf16_bits_rne(x);
+                l[q] = bf16_bits_rne(x - bf16_to_float(h[q]));
+            }
 #pragma unroll
-            for (uint32_t e = 0; e < 4; ++e) {
-                const uint16_t h = bf16_bits_rne(xs[e]);
-                a_hi[(m4 + e) * kLdsPitch + kr] = h;
-                a_lo[(m4 + e) * kLdsPitch + kr] = bf16_bits_rne(xs[e] - bf16_to_float(h));
+            for (uint32_t q = 0; q < 2; ++q) {
+                uint4 vh, vl;
+                vh.x = h[8 * q] | ((uint32_t)h[8 * q + 1] << 16);
+                vh.y = h[8 * q + 2] | ((uint32_t)h[8 * q + 3] << 16);
+                vh.z = h[8 * q + 4] | ((uint32_t)h[8 * q + 5] << 16);
+                vh.w = h[8 * q + 6] | ((uint32_t)h[8 * q + 7] << 16);
+                vl.x = l[8 * q] | ((uint32_t)l[8 * q + 1] << 16);
+                vl.y = l[8 * q + 2] | ((uint32_t)l[8 * q + 3] << 16);
+                vl.z = l[8 * q + 4] | ((uint32_t)l[8 * q + 5] << 16);
+                vl.w = l[8 * q + 6] | ((uint32_t)l[8 * q + 7] << 16);
+                *reinterpret_cast<uint4 *>(&a_hi[m * kLdsPitch + kg + 8 * q]) = vh;
+                *reinterpret_cast<uint4 *>(&a_lo[m * kLdsPitch + kg + 8 * q]) = vl;
             }
         }
-        // waveform slice: 256 rows x 32 k (bf16, k contiguous), 64 bytes a row
+        // waveform slice: 128 rows x 32 k (bf16, k contiguous), 64 bytes a row
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
+        for (uint32_t q = 0; q < 2; ++q) {
             const uint32_t c = t + 256u * q;  // 16-byte piece: row c/4, quarter c%4
             const uint32_t row = c >> 2, part = (c & 3u) * 8u;
             const size_t src = (size_t)(n0 + row) * k_total + k0 + ks + part;
@@ -1089,7 +1109,7 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
 #pragma unroll
         for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
             const uint32_t kofs = kk + 8u * (lane >> 5);
-            bf16x8 ah[2], al[2], bh[4], bl[4];
+            bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const uint32_t row = wm + 32u * i + (lane & 31u);
@@ -1097,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
                 al[i] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * kLdsPitch + kofs]);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < 2; ++j) {
                 const uint32_t row = wn + 32u * j + (lane & 31u);
                 bh[j] = *reinterpret_cast<const bf16x8 *>(&b_hi[row * kLdsPitch + kofs]);
                 bl[j] = *reinterpret_cast<const bf16x8 *>(&b_lo[row * kLdsPitch + kofs]);
@@ -1105,47 +1125,53 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 2; ++j) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
                 }
         }
     }
-    // Kahan fold into phi / comp ([spin][waveform]); lanes 0-31 and 32-63
-    // each store 32 consecutive waveforms of one spin
+    // Kahan fold into (phase, compensation) pairs [spin][waveform]; lanes 0-31
+    // and 32-63 each update 32 consecutive waveforms of one spin
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j) {
+            // all 16 loads of an MFMA tile in flight before any arithmetic
+            const uint32_t w = n0 + wn + 32u * j + (lane & 31u);
+            const uint32_t sb = m0 + wm + 32u * i + 4u * (lane >> 5);
+            float2 pc[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const uint32_t s = m0 + wm + 32u * i + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
-                const uint32_t w = n0 + wn + 32u * j + (lane & 31u);
-                if (s >= n) continue;
-                const size_t o = (size_t)s * n_wave + w;
-                const float p = phi[o], c = comp[o];
-                const float y = acc[i][j][r] - c;
-                const float tt = p + y;
-                comp[o] = (tt - p) - y;
-                phi[o] = tt;
+                const uint32_t s = sb + 8u * (r / 4) + (r % 4);
+                pc[r] = s < n ? phi[(size_t)s * n_wave + w] : make_float2(0.0f, 0.0f);
             }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t s = sb + 8u * (r / 4) + (r % 4);
+                if (s >= n) continue;
+                const float y = acc[i][j][r] - pc[r].y;
+                const float tt = pc[r].x + y;
+                phi[(size_t)s * n_wave + w] = make_float2(tt, (tt - pc[r].x) - y);
+            }
+        }
 }
 
 hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
                              const uint16_t *g_hi, const uint16_t *g_lo, uint32_t k_total,
-                             uint32_t k0, uint32_t n_wave, float *phi, float *comp,
-                             hipStream_t s) {
+                             uint32_t k0, uint32_t n_wave, float2 *phi, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    dim3 grid(n_wave / kPhaseTileN, n_pad / kPhaseTileM);
-    hipLaunchKernelGGL(k_phase_gemm, grid, dim3(256), 0, s, traj, n, n_pad, k_chunk, g_hi, g_lo,
-                       k_total, k0, n_wave, phi, comp);
+    const uint32_t m_groups = (n_pad / kPhaseTileM + 7u) / 8u;
+    const uint32_t grid = m_groups * 8u * (n_wave / kPhaseTileN);
+    hipLaunchKernelGGL(k_phase_gemm, dim3(grid), dim3(256), 0, s, traj, n, n_pad, k_chunk, g_hi,
+                       g_lo, k_total, k0, n_wave, phi);
     return hipGetLastError();
 }
 
 // Readout of the waveform phases: one thread per waveform, one block row per
 // range of spins; FP64 phase (x0 term added in FP64), cos/sin and sums.
-__global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *__restrict__ phi,
+__global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float2 *__restrict__ phi,
                                                        uint32_t n_wave,
                                                        const double *__restrict__ gsum,
                                                        const float *__restrict__ mag,
@@ -1160,7 +1186,7 @@ __global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *
     for (int k = 0; k < 15; ++k) v[k] = 0.0;
     const uint32_t s1 = min(n, (sb + 1) * per_block);
     for (uint32_t s = sb * per_block; s < s1; ++s) {
-        const double ph = (double)phi[(size_t)s * n_wave + w] + gx * (double)x0[s] +
+        const double ph = (double)phi[(size_t)s * n_wave + w].x + gx * (double)x0[s] +
                           gy * (double)x0[n + s] + gz * (double)x0[2 * n + s];
         double sn, cs;
         sincos(ph, &sn, &cs);
@@ -1182,7 +1208,7 @@ __global__ __launch_bounds__(256) void k_phase_final(const double *partial, uint
     out[i] = acc;
 }
 
-hipError_t launch_phase_signal(uint32_t n, const float *phi, uint32_t n_wave,
+hipError_t launch_phase_signal(uint32_t n, const float2 *phi, uint32_t n_wave,
                                const double *gsum, const float *mag, const uint32_t *label,
                                const float *x0, double *partial, uint32_t n_blocks,
                                double *out, hipStream_t s) {

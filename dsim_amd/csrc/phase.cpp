@@ -25,7 +25,7 @@ struct dsim_waveforms {
     DevBuf<uint16_t> g_hi, g_lo;  // [n_wave][3*n_steps] bf16 bits, pc folded in
     DevBuf<double> gsum;          // [n_wave][3]: sum_t pc * G_w(t)
     DevBuf<float> traj;           // [3*chunk][n_pad]
-    DevBuf<float> phi, comp;      // [n][n_wave]
+    DevBuf<float2> phi;           // [n][n_wave]: (phase, Kahan compensation)
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
 };
@@ -57,8 +57,8 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     *out = nullptr;
     if (n_wave == 0 || n_wave % dsim::kPhaseTileN)
         return fail(DSIM_ERR_ARG, "n_wave must be a positive multiple of %u", dsim::kPhaseTileN);
-    if (chunk == 0 || chunk % dsim::kPhaseSliceK || n_steps == 0 || n_steps % chunk)
-        return fail(DSIM_ERR_ARG, "chunk_steps must be a multiple of %u dividing n_steps",
+    if (chunk == 0 || chunk % dsim::kPhaseSliceK || n_steps == 0 || n_steps % dsim::kPhaseSliceK)
+        return fail(DSIM_ERR_ARG, "chunk_steps and n_steps must be multiples of %u",
                     dsim::kPhaseSliceK);
     auto *W = new dsim_waveforms();
     W->sim = sim;
@@ -83,7 +83,6 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     const size_t nw = (size_t)std::max<uint32_t>(W->n, 1) * n_wave;
     if ((rc = upload(W->g_hi, hi)) || (rc = upload(W->g_lo, lo)) || (rc = upload(W->gsum, gs)) ||
         (rc = W->traj.alloc((size_t)3 * chunk * W->n_pad)) || (rc = W->phi.alloc(nw)) ||
-        (rc = W->comp.alloc(nw)) ||
         (rc = W->partial.alloc((size_t)kPhaseBlocks * n_wave * 15)) ||
         (rc = W->out.alloc((size_t)n_wave * 15))) {
         delete W;
@@ -91,8 +90,7 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     }
     // padded trajectory columns stay zero; phases start at zero
     hipError_t e = hipMemset(W->traj.p, 0, W->traj.n * 4);
-    if (e == hipSuccess) e = hipMemset(W->phi.p, 0, nw * 4);
-    if (e == hipSuccess) e = hipMemset(W->comp.p, 0, nw * 4);
+    if (e == hipSuccess) e = hipMemset(W->phi.p, 0, nw * 8);
     if (e != hipSuccess) {
         delete W;
         return fail(DSIM_ERR_HIP, "waveform init: %s", hipGetErrorString(e));
@@ -110,27 +108,28 @@ extern "C" int dsim_waveforms_destroy(dsim_waveforms *W) {
 
 extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj_out) {
     if (!W) return fail(DSIM_ERR_ARG, "NULL waveforms");
-    if (steps % W->chunk || W->done + steps > W->n_steps)
+    if (steps % dsim::kPhaseSliceK || W->done + steps > W->n_steps)
         return fail(DSIM_ERR_ARG, "steps must be a multiple of %u within the waveforms' %u",
-                    W->chunk, W->n_steps);
+                    dsim::kPhaseSliceK, W->n_steps);
     hipStream_t st = dsim_host::sim_stream(W->sim);
     hipEvent_t ev[3];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     int rc = DSIM_OK;
-    for (uint32_t c = 0; c < steps && rc == DSIM_OK; c += W->chunk) {
+    for (uint32_t c = 0; c < steps && rc == DSIM_OK;) {
+        const uint32_t len = std::min(W->chunk, steps - c);  // a multiple of 32
         HIP_TRY(hipEventRecord(ev[0], st));
-        if ((rc = dsim_host::sim_walk_traj(W->sim, W->chunk, W->traj.p, W->n_pad))) break;
+        if ((rc = dsim_host::sim_walk_traj(W->sim, len, W->traj.p, W->n_pad))) break;
         HIP_TRY(hipEventRecord(ev[1], st));
-        hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * W->chunk,
+        hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len,
                                                W->g_hi.p, W->g_lo.p, 3 * W->n_steps,
-                                               3 * W->done, W->n_wave, W->phi.p, W->comp.p, st);
+                                               3 * W->done, W->n_wave, W->phi.p, st);
         if (e != hipSuccess) {
             rc = fail(DSIM_ERR_HIP, "phase GEMM: %s", hipGetErrorString(e));
             break;
         }
         HIP_TRY(hipEventRecord(ev[2], st));
         if (traj_out) {
-            for (uint32_t r = 0; r < 3 * W->chunk; ++r)
+            for (uint32_t r = 0; r < 3 * len; ++r)
                 HIP_TRY(hipMemcpyAsync(traj_out + ((size_t)3 * c + r) * W->n,
                                        W->traj.p + (size_t)r * W->n_pad, (size_t)W->n * 4,
                                        hipMemcpyDeviceToHost, st));
@@ -141,7 +140,8 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
         HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
         W->walk_ms += a;
         W->gemm_ms += b;
-        W->done += W->chunk;
+        W->done += len;
+        c += len;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
@@ -173,9 +173,13 @@ extern "C" int dsim_waveforms_signal(dsim_waveforms *W, double *out) {
 extern "C" int dsim_waveforms_get_phase(dsim_waveforms *W, float *phase, float *comp) {
     if (!W) return fail(DSIM_ERR_ARG, "NULL waveforms");
     if (int rc = dsim_sync(W->sim)) return rc;
-    const size_t bytes = (size_t)W->n * W->n_wave * 4;
-    if (phase) HIP_TRY(hipMemcpy(phase, W->phi.p, bytes, hipMemcpyDeviceToHost));
-    if (comp) HIP_TRY(hipMemcpy(comp, W->comp.p, bytes, hipMemcpyDeviceToHost));
+    const size_t m = (size_t)W->n * W->n_wave;
+    std::vector<float2> v(m);
+    if (m) HIP_TRY(hipMemcpy(v.data(), W->phi.p, m * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < m; ++k) {
+        if (phase) phase[k] = v[k].x;
+        if (comp) comp[k] = v[k].y;
+    }
     return DSIM_OK;
 }
 

@@ -30,8 +30,9 @@ extern "C" {
 
 typedef struct dsim_waveforms dsim_waveforms;
 
-/* n_wave: a multiple of 256; n_steps: a multiple of chunk_steps; chunk_steps:
- * a multiple of 32 (64 gives the GEMM K = 192); grads: host array
+/* n_wave: a multiple of 128; n_steps and chunk_steps: multiples of 32 (a
+ * chunk of 64 steps gives the GEMM K = 192; the last chunk may be shorter);
+ * grads: host array
  * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
  * phase_constant = 2*pi*gamma*dt.  Device memory: 8 bytes per (spin,
  * waveform) for the phase and its compensation, plus 4 * n_wave * 3 * n_steps
@@ -39,8 +40,8 @@ typedef struct dsim_waveforms dsim_waveforms;
 int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_steps, const float *grads,
                           float phase_constant, uint32_t chunk_steps, dsim_waveforms **out);
 int dsim_waveforms_destroy(dsim_waveforms *wf);
-/* Walks `steps` more steps (a multiple of chunk_steps; the total may not pass
- * n_steps) and accumulates the phases.  traj_out (may be NULL): host array
+/* Walks `steps` more steps (a multiple of 32; the total may not pass n_steps)
+ * and accumulates the phases, chunk_steps at a time.  traj_out (may be NULL): host array
  * [steps][3][n_spins] that receives the recorded displacements (for checks). */
 int dsim_waveforms_run(dsim_waveforms *wf, uint32_t steps, float *traj_out);
 /* Signals per waveform, [n_wave*5]: total, then compartments 0-3, as
