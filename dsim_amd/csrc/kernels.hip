@@ -1040,10 +1040,14 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
                                                     const uint16_t *__restrict__ g_lo,
                                                     uint32_t k_total, uint32_t k0,
                                                     uint32_t n_wave, float2 *__restrict__ phi) {
-    __shared__ __align__(16) uint16_t a_hi[kPhaseTileM * kLdsPitch];
-    __shared__ __align__(16) uint16_t a_lo[kPhaseTileM * kLdsPitch];
-    __shared__ __align__(16) uint16_t b_hi[kPhaseTileN * kLdsPitch];
-    __shared__ __align__(16) uint16_t b_lo[kPhaseTileN * kLdsPitch];
+    // K loop: bf16 hi/lo operand slices; epilogue: the same memory holds half
+    // of the FP32 result tile (64 spins x 128 waveforms)
+    constexpr uint32_t kOperandHalves = kPhaseTileM * kLdsPitch;
+    __shared__ __align__(16) uint16_t lds[2 * kOperandHalves + 2 * kPhaseTileN * kLdsPitch];
+    static_assert(sizeof(lds) >= 64 * kPhaseTileN * sizeof(float), "epilogue stage must fit");
+    uint16_t *a_hi = lds, *a_lo = lds + kOperandHalves;
+    uint16_t *b_hi = lds + 2 * kOperandHalves, *b_lo = b_hi + kPhaseTileN * kLdsPitch;
+    float *stage = reinterpret_cast<float *>(lds);
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     // Workgroups go to the 8 XCDs in turn; the numbering below hands all
     // waveform tiles of one spin tile to one XCD back to back, so the
@@ -1132,30 +1136,45 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
                 }
         }
     }
-    // Kahan fold into (phase, compensation) pairs [spin][waveform]; lanes 0-31
-    // and 32-63 each update 32 consecutive waveforms of one spin
+    // Kahan fold into (phase, compensation) pairs [spin][waveform], one half of
+    // the tile (64 spins) at a time through LDS: the two waves holding those
+    // spins stage their results, then every thread updates 16 pieces of 16
+    // bytes (two waveforms each), a wave covering one 1 KB row per access,
+    // with all 16 loads in flight before any arithmetic.
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (uint32_t h = 0; h < 2; ++h) {
+        __syncthreads();
+        if (wm == 64u * h) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            // all 16 loads of an MFMA tile in flight before any arithmetic
-            const uint32_t w = n0 + wn + 32u * j + (lane & 31u);
-            const uint32_t sb = m0 + wm + 32u * i + 4u * (lane >> 5);
-            float2 pc[16];
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t s = sb + 8u * (r / 4) + (r % 4);
-                pc[r] = s < n ? phi[(size_t)s * n_wave + w] : make_float2(0.0f, 0.0f);
-            }
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t s = sb + 8u * (r / 4) + (r % 4);
-                if (s >= n) continue;
-                const float y = acc[i][j][r] - pc[r].y;
-                const float tt = pc[r].x + y;
-                phi[(size_t)s * n_wave + w] = make_float2(tt, (tt - pc[r].x) - y);
-            }
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t row = 32u * i + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
+                        stage[row * kPhaseTileN + wn + 32u * j + (lane & 31u)] = acc[i][j][r];
+                    }
         }
+        __syncthreads();
+        const uint32_t c = t & 63u, r0 = t >> 6;
+        float4 v[16];
+#pragma unroll
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t s = m0 + 64u * h + r0 + 4u * q;
+            v[q] = s < n ? *reinterpret_cast<const float4 *>(phi + (size_t)s * n_wave + n0 + 2u * c)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t rr = r0 + 4u * q, s = m0 + 64u * h + rr;
+            if (s >= n) continue;
+            const float2 d = *reinterpret_cast<const float2 *>(&stage[rr * kPhaseTileN + 2u * c]);
+            const float y0 = d.x - v[q].y, t0 = v[q].x + y0;
+            const float y1 = d.y - v[q].w, t1 = v[q].z + y1;
+            *reinterpret_cast<float4 *>(phi + (size_t)s * n_wave + n0 + 2u * c) =
+                make_float4(t0, (t0 - v[q].x) - y0, t1, (t1 - v[q].z) - y1);
+        }
+    }
 }
 
 hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
