@@ -335,8 +335,16 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
         float nd = std::fmaf(h[0], v0x, std::fmaf(h[1], v0y, h[2] * v0z));
         mt[t * 4 + 0] = make_float4(h[0], h[1], h[2], nd);
         mt[t * 4 + 1] = make_float4(v0x, v0y, v0z, bits_to_float(info));
-        mt[t * 4 + 2] = make_float4(V[b * 3] - v0x, V[b * 3 + 1] - v0y, V[b * 3 + 2] - v0z, 0.f);
-        mt[t * 4 + 3] = make_float4(V[c * 3] - v0x, V[c * 3 + 1] - v0y, V[c * 3 + 2] - v0z, 0.f);
+        const float e1x = V[b * 3] - v0x, e1y = V[b * 3 + 1] - v0y, e1z = V[b * 3 + 2] - v0z;
+        const float e2x = V[c * 3] - v0x, e2y = V[c * 3 + 1] - v0y, e2z = V[c * 3 + 2] - v0z;
+        // 1e-6 |e1 x e2|^2: a segment d with det^2 < this * |d|^2 lies within
+        // about 1e-3 rad of the plane and is re-tested in FP64
+        const float cx = std::fmaf(e1y, e2z, -(e1z * e2y));
+        const float cy = std::fmaf(e1z, e2x, -(e1x * e2z));
+        const float cz = std::fmaf(e1x, e2y, -(e1y * e2x));
+        const float graze = 1e-6f * std::fmaf(cx, cx, std::fmaf(cy, cy, cz * cz));
+        mt[t * 4 + 2] = make_float4(e1x, e1y, e1z, graze);
+        mt[t * 4 + 3] = make_float4(e2x, e2y, e2z, 0.f);
         float mn[3], mx[3];
         for (int k = 0; k < 3; ++k) {
             float p = V[a * 3 + k], q = V[b * 3 + k], r = V[c * 3 + k];

@@ -160,37 +160,97 @@ __device__ __forceinline__ void test_ref(const float4 *__restrict__ R, uint32_t 
     }
 }
 
-// Plane-side prefilter: does the segment o-p cross the plane (n, n.v0)?
+// A segment within about 1e-3 rad of a triangle's plane, det^2 < 1e-6
+// |e1 x e2|^2 |d|^2, is decided in FP64; the triangle record carries
+// 1e-6 |e1 x e2|^2 in e1.w.  The plane prefilter keeps every segment whose end
+// points' signed distances s0, s1 (normalised units) have s0 s1 <= kCrossTol:
+// all true crossings, and same-side segments whose rounded distances could
+// belong to a crossing -- for a crossing within 1e-3 rad, one of s0, s1 is
+// within the distance's rounding error (below 4e-7 for coordinates within
+// [-1, 1]) of zero and the other below 1e-3 times the step length.
+constexpr float kCrossTol = 1e-9f;
+
+// Plane-side prefilter: can the segment o-p cross the plane (n, n.v0)?
 __device__ __forceinline__ bool crosses_plane(const float4 pl, float ox, float oy, float oz,
                                               float px, float py, float pz) {
     float s0 = fmaf(pl.x, ox, fmaf(pl.y, oy, fmaf(pl.z, oz, -pl.w)));
     float s1 = fmaf(pl.x, px, fmaf(pl.y, py, fmaf(pl.z, pz, -pl.w)));
-    return (s0 > 0.0f) != (s1 > 0.0f);
+    return s0 * s1 <= kCrossTol;
 }
 
-// Moller-Trumbore on the open segment 0<t<1 for triangle t with first vertex
-// v0 and edges e1, e2 (xyz), whose plane the segment crosses.
-__device__ __forceinline__ void mt_core(const float4 v0, const float4 e1, const float4 e2,
+__device__ __forceinline__ float seg_len2(float ox, float oy, float oz, float px, float py,
+                                          float pz) {
+    const float dx = px - ox, dy = py - oy, dz = pz - oz;
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+// Moller-Trumbore in FP64 for a near-grazing segment.  The differences of
+// float coordinates are exact in double; the inside tests compare the
+// numerators u*det, v*det, t*det with det after making det positive, so
+// they involve no division; t and the hit point are rounded back to float.
+__device__ __forceinline__ void mt_fp64(const float4 v0, const float4 e1, const float4 e2,
                                         uint32_t t, float ox, float oy, float oz, float px,
                                         float py, float pz, Hit &h) {
+    const double Dx = (double)px - (double)ox, Dy = (double)py - (double)oy,
+                 Dz = (double)pz - (double)oz;
+    const double Tx = (double)ox - (double)v0.x, Ty = (double)oy - (double)v0.y,
+                 Tz = (double)oz - (double)v0.z;
+    const double Qx = Dy * (double)e2.z - Dz * (double)e2.y;
+    const double Qy = Dz * (double)e2.x - Dx * (double)e2.z;
+    const double Qz = Dx * (double)e2.y - Dy * (double)e2.x;
+    double det = (double)e1.x * Qx + (double)e1.y * Qy + (double)e1.z * Qz;
+    if (det == 0.0) return;
+    const double s = det < 0.0 ? -1.0 : 1.0;
+    det *= s;
+    const double un = s * (Tx * Qx + Ty * Qy + Tz * Qz);
+    if (!(un >= 0.0 && un <= det)) return;
+    const double Cx = Ty * (double)e1.z - Tz * (double)e1.y;
+    const double Cy = Tz * (double)e1.x - Tx * (double)e1.z;
+    const double Cz = Tx * (double)e1.y - Ty * (double)e1.x;
+    const double vn = s * (Dx * Cx + Dy * Cy + Dz * Cz);
+    if (!(vn >= 0.0 && un + vn <= det)) return;
+    const double tn = s * ((double)e2.x * Cx + (double)e2.y * Cy + (double)e2.z * Cz);
+    if (!(tn > 0.0 && tn < det)) return;
+    const double tt = tn / det;
+    h.px = (float)((double)ox + tt * Dx);
+    h.py = (float)((double)oy + tt * Dy);
+    h.pz = (float)((double)oz + tt * Dz);
+    h.d = (float)tt;
+    h.idx = t;
+    h.type = 1;
+    if (un == 0.0 || vn == 0.0 || un + vn == det) {
+        h.type = 2;
+        if ((un == 0.0 && vn == 0.0) || (un == det && vn == 0.0) || (un == 0.0 && vn == det))
+            h.type = 3;
+    }
+}
+
+// Moller-Trumbore in FP32 on the open segment 0<t<1 for triangle t with first
+// vertex v0 and edges e1, e2 (e1.w: the grazing scale), whose plane the
+// segment may cross; d2 = |p - o|^2 (seg_len2).  Returns true, with no hit,
+// for a near-grazing segment, which mt_fp64 decides.
+__device__ __forceinline__ bool mt_fp32(const float4 v0, const float4 e1, const float4 e2,
+                                        uint32_t t, float ox, float oy, float oz, float px,
+                                        float py, float pz, float d2, Hit &h) {
     h.type = 0;
     float dx = px - ox, dy = py - oy, dz = pz - oz;
     float qx_ = fmaf(dy, e2.z, -(dz * e2.y));
     float qy_ = fmaf(dz, e2.x, -(dx * e2.z));
     float qz_ = fmaf(dx, e2.y, -(dy * e2.x));
     float det = fmaf(e1.x, qx_, fmaf(e1.y, qy_, e1.z * qz_));
-    if (det == 0.0f) return;
+    const bool graze = det * det < e1.w * d2;
+    if (graze || det == 0.0f) return graze;
     float inv = 1.0f / det;
     float tx = ox - v0.x, ty = oy - v0.y, tz = oz - v0.z;
     float u = fmaf(tx, qx_, fmaf(ty, qy_, tz * qz_)) * inv;
-    if (!(u >= 0.0f && u <= 1.0f)) return;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
     float cx = fmaf(ty, e1.z, -(tz * e1.y));
     float cy = fmaf(tz, e1.x, -(tx * e1.z));
     float cz = fmaf(tx, e1.y, -(ty * e1.x));
     float v = fmaf(dx, cx, fmaf(dy, cy, dz * cz)) * inv;
-    if (!(v >= 0.0f && u + v <= 1.0f)) return;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
     float tt = fmaf(e2.x, cx, fmaf(e2.y, cy, e2.z * cz)) * inv;
-    if (!(tt > 0.0f && tt < 1.0f)) return;
+    if (!(tt > 0.0f && tt < 1.0f)) return false;
     h.px = fmaf(tt, dx, ox);
     h.py = fmaf(tt, dy, oy);
     h.pz = fmaf(tt, dz, oz);
@@ -202,6 +262,15 @@ __device__ __forceinline__ void mt_core(const float4 v0, const float4 e1, const 
         if ((u == 0.0f && v == 0.0f) || (u == 1.0f && v == 0.0f) || (u == 0.0f && v == 1.0f))
             h.type = 3;
     }
+    return false;
+}
+
+// Moller-Trumbore with the FP64 re-test of near-grazing segments.
+__device__ __forceinline__ void mt_core(const float4 v0, const float4 e1, const float4 e2,
+                                        uint32_t t, float ox, float oy, float oz, float px,
+                                        float py, float pz, Hit &h) {
+    if (mt_fp32(v0, e1, e2, t, ox, oy, oz, px, py, pz, seg_len2(ox, oy, oz, px, py, pz), h))
+        mt_fp64(v0, e1, e2, t, ox, oy, oz, px, py, pz, h);
 }
 
 // The same test on the triangle's record in the MT table.
@@ -568,15 +637,16 @@ __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
     return ((unsigned long long)__float_as_uint(t) << 32) | tri;
 }
 
-// Triangle test of queue entry q (cell-major record) for its lane; the
-// lane's minimum key in best[] is lowered to it on a hit.
-__device__ __forceinline__ void queue_test(const StepParams &P, WaveScratch &L, uint32_t q,
+// FP32 triangle test of queue entry q (cell-major record) for its lane; the
+// lane's minimum key in best[] is lowered to it on a hit.  Returns true for a
+// near-grazing entry, which wave_graze_pass decides.
+__device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, uint32_t q,
                                            uint32_t qlen, Hit &h, uint32_t &owner,
                                            unsigned long long &key) {
     h.type = 0;
     owner = 0;
     key = ~0ull;
-    if (q >= qlen) return;
+    if (q >= qlen) return false;
     const uint2 e = L.queue[q];
     owner = e.y;
     const float4 v0 = P.cell_mt[(size_t)e.x * 3 + 0];
@@ -584,11 +654,47 @@ __device__ __forceinline__ void queue_test(const StepParams &P, WaveScratch &L, 
     const float4 e2 = P.cell_mt[(size_t)e.x * 3 + 2];
     const uint32_t t = __float_as_uint(v0.w);
     const float4 so = L.o[owner], sp = L.p[owner];
-    if (t == __float_as_uint(so.w)) return;
-    mt_core(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
+    if (t == __float_as_uint(so.w)) return false;
+    if (mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) return true;
     if (h.type) {
         key = hit_key(h.d, t);
         atomicMin(&L.best[owner], key);
+    }
+    return false;
+}
+
+// FP64 re-test of the near-grazing queue entries in [0, qlen) (found again
+// by the FP32 test), one entry per lane per trip, with the nearest-hit
+// bookkeeping of the FP32 pass.  Wave-uniform qlen.
+__device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch &L,
+                                                uint32_t lane, uint32_t qlen) {
+    for (uint32_t qb = 0; qb < qlen; qb += 64) {
+        wave_lds_sync();
+        Hit h;
+        h.type = 0;
+        uint32_t owner = 0;
+        unsigned long long key = ~0ull;
+        const uint32_t q = qb + lane;
+        if (q < qlen) {
+            const uint2 e = L.queue[q];
+            const float4 v0 = P.cell_mt[(size_t)e.x * 3 + 0];
+            const float4 e1 = P.cell_mt[(size_t)e.x * 3 + 1];
+            const float4 e2 = P.cell_mt[(size_t)e.x * 3 + 2];
+            const uint32_t t = __float_as_uint(v0.w);
+            const float4 so = L.o[e.y], sp = L.p[e.y];
+            if (t != __float_as_uint(so.w) &&
+                mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) {
+                mt_fp64(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
+                if (h.type) {
+                    owner = e.y;
+                    key = hit_key(h.d, t);
+                    atomicMin(&L.best[owner], key);
+                }
+            }
+        }
+        wave_lds_sync();
+        if (h.type && L.best[owner] == key)
+            L.hit[owner] = make_float4(h.px, h.py, h.pz, __uint_as_float(h.type));
     }
 }
 
@@ -596,13 +702,14 @@ __device__ __forceinline__ void queue_test(const StepParams &P, WaveScratch &L, 
 // point into hit[].  Wave-uniform qlen.
 __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScratch &L,
                                                    uint32_t lane, uint32_t qlen) {
+    bool graze = false;
     for (uint32_t qb = 0; qb < qlen; qb += kPass) {
         wave_lds_sync();
         Hit ha, hb;
         uint32_t oa, ob;
         unsigned long long ka, kb;
-        queue_test(P, L, qb + lane, qlen, ha, oa, ka);
-        queue_test(P, L, qb + 64 + lane, qlen, hb, ob, kb);
+        graze |= queue_test(P, L, qb + lane, qlen, ha, oa, ka);
+        graze |= queue_test(P, L, qb + 64 + lane, qlen, hb, ob, kb);
         wave_lds_sync();
         // the pair holding the lane's minimum so far records its hit point; a
         // later pass that lowers the minimum overwrites it
@@ -611,6 +718,7 @@ __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScra
         if (hb.type && L.best[ob] == kb)
             L.hit[ob] = make_float4(hb.px, hb.py, hb.pz, __uint_as_float(hb.type));
     }
+    if (__builtin_expect(__ballot(graze) != 0, 0)) wave_graze_pass(P, L, lane, qlen);
     wave_lds_sync();
 }
 
@@ -690,7 +798,7 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
     const uint32_t total = __builtin_amdgcn_readlane(cend, 63);
     if (coop) {
         L.o[lane] = make_float4(ox, oy, oz, __uint_as_float(excluded));
-        L.p[lane] = make_float4(px, py, pz, 0.0f);
+        L.p[lane] = make_float4(px, py, pz, seg_len2(ox, oy, oz, px, py, pz));
         L.meta[lane] = make_uint2(off, soff);
         L.best[lane] = ~0ull;
     }
@@ -927,7 +1035,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
 }
 
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
-__global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[4];
     walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(), iters, scratch[threadIdx.x >> 6]);
 }
@@ -940,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_walk(StepParams P, uint32_t iters) {
 // table's total exit at once, so one launch shape (and one captured graph)
 // serves any mix of jobs.
 template <bool PERIODIC>
-__global__ __launch_bounds__(256) void k_walk_batch(const WalkTable *__restrict__ T) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_batch(const WalkTable *__restrict__ T) {
     __shared__ WaveScratch scratch[4];
     const uint32_t b = xcd_block();
     if (b >= T->total_blocks) return;

@@ -141,8 +141,16 @@ void orc_tri_helpers(const float *V, const uint32_t *T, uint32_t n_tri, float *H
     }
 }
 
-/* Build-defined 64-byte Moller-Trumbore record: plane (n, n.v0), v0, e1=v1-v0,
- * e2=v2-v0.  The normal is the reference helper normal. */
+/* Build-defined tolerances of the production test (orc_mt_coll_detect): the
+ * plane prefilter keeps segments whose end points' signed plane distances
+ * have s0 s1 <= ORC_CROSS_TOL; segments within about 1e-3 rad of the plane
+ * (det^2 < ORC_GRAZE_SQ |e1 x e2|^2 |d|^2) are decided in FP64. */
+#define ORC_CROSS_TOL 1e-9f
+#define ORC_GRAZE_SQ 1e-6f
+
+/* Build-defined 64-byte Moller-Trumbore record: plane (n, n.v0), v0, e1=v1-v0
+ * with ORC_GRAZE_SQ |e1 x e2|^2 in the fourth slot, e2=v2-v0.  The normal is
+ * the reference helper normal. */
 void orc_tri_mt_records(const float *V, const uint32_t *T, uint32_t n_tri, float *R)
 {
     for (uint32_t i = 0; i < n_tri; ++i) {
@@ -154,8 +162,13 @@ void orc_tri_mt_records(const float *V, const uint32_t *T, uint32_t n_tri, float
         r[0] = h[0]; r[1] = h[1]; r[2] = h[2];
         r[3] = fmaf(h[0], v0x, fmaf(h[1], v0y, h[2] * v0z));
         r[4] = v0x; r[5] = v0y; r[6] = v0z; r[7] = 0.0f;
-        r[8] = V[b * 3 + 0] - v0x; r[9] = V[b * 3 + 1] - v0y; r[10] = V[b * 3 + 2] - v0z; r[11] = 0.0f;
+        r[8] = V[b * 3 + 0] - v0x; r[9] = V[b * 3 + 1] - v0y; r[10] = V[b * 3 + 2] - v0z;
         r[12] = V[c * 3 + 0] - v0x; r[13] = V[c * 3 + 1] - v0y; r[14] = V[c * 3 + 2] - v0z; r[15] = 0.0f;
+        /* ORC_GRAZE_SQ |e1 x e2|^2, the near-grazing scale of det^2 / |d|^2 */
+        float cx = fmaf(r[9], r[14], -(r[10] * r[13]));
+        float cy = fmaf(r[10], r[12], -(r[8] * r[14]));
+        float cz = fmaf(r[8], r[13], -(r[9] * r[12]));
+        r[11] = ORC_GRAZE_SQ * fmaf(cx, cx, fmaf(cy, cy, cz * cz));
     }
 }
 
@@ -379,9 +392,48 @@ orc_coll orc_tri_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tr
     return res;
 }
 
-/* Build-defined production test: plane-side prefilter, then Moller-Trumbore
+/* Build-defined.  Moller-Trumbore in FP64 for a near-grazing segment.  The
+ * differences of float coordinates are exact in double; the inside tests
+ * compare the numerators u*det, v*det, t*det with det after making det
+ * positive (no division); t and the hit point are rounded back to float. */
+static orc_coll mt_fp64(const float *r, orc_f3 o, orc_f3 p, uint32_t tri, orc_coll res)
+{
+    const double Dx = (double)p.x - (double)o.x, Dy = (double)p.y - (double)o.y,
+                 Dz = (double)p.z - (double)o.z;
+    const double Tx = (double)o.x - (double)r[4], Ty = (double)o.y - (double)r[5],
+                 Tz = (double)o.z - (double)r[6];
+    const double E1x = r[8], E1y = r[9], E1z = r[10], E2x = r[12], E2y = r[13], E2z = r[14];
+    const double Qx = Dy * E2z - Dz * E2y, Qy = Dz * E2x - Dx * E2z, Qz = Dx * E2y - Dy * E2x;
+    double det = E1x * Qx + E1y * Qy + E1z * Qz;
+    if (det == 0.0) return res;
+    const double s = det < 0.0 ? -1.0 : 1.0;
+    det *= s;
+    const double un = s * (Tx * Qx + Ty * Qy + Tz * Qz);
+    if (!(un >= 0.0 && un <= det)) return res;
+    const double Cx = Ty * E1z - Tz * E1y, Cy = Tz * E1x - Tx * E1z, Cz = Tx * E1y - Ty * E1x;
+    const double vn = s * (Dx * Cx + Dy * Cy + Dz * Cz);
+    if (!(vn >= 0.0 && un + vn <= det)) return res;
+    const double tn = s * (E2x * Cx + E2y * Cy + E2z * Cz);
+    if (!(tn > 0.0 && tn < det)) return res;
+    const double t = tn / det;
+    res.pt = f3((float)((double)o.x + t * Dx), (float)((double)o.y + t * Dy),
+                (float)((double)o.z + t * Dz));
+    res.dist_sq = (float)t;
+    res.idx = tri;
+    res.type = 1;
+    if (un == 0.0 || vn == 0.0 || un + vn == det) {
+        res.type = 2;
+        if ((un == 0.0 && vn == 0.0) || (un == det && vn == 0.0) || (un == 0.0 && vn == det))
+            res.type = 3;
+    }
+    return res;
+}
+
+/* Build-defined production test: plane-side prefilter (s0 s1 <= ORC_CROSS_TOL),
+ * then Moller-Trumbore
  * ("Fast, minimum storage ray/triangle intersection", JGT 1997) restricted to
- * the open segment 0<t<1.  dist_sq carries t. */
+ * the open segment 0<t<1, in FP64 when the segment is within about 1e-3 rad
+ * of the plane (det^2 < ORC_GRAZE_SQ |e1 x e2|^2 |d|^2).  dist_sq carries t. */
 orc_coll orc_mt_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri)
 {
     orc_coll res;
@@ -392,7 +444,7 @@ orc_coll orc_mt_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri
     const float *r = w->tri_mt + (size_t)tri * 16;
     float s0 = fmaf(r[0], o.x, fmaf(r[1], o.y, fmaf(r[2], o.z, -r[3])));
     float s1 = fmaf(r[0], p.x, fmaf(r[1], p.y, fmaf(r[2], p.z, -r[3])));
-    if ((s0 > 0.0f) == (s1 > 0.0f)) return res;
+    if (s0 * s1 > ORC_CROSS_TOL) return res;
     float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
     float e1x = r[8], e1y = r[9], e1z = r[10];
     float e2x = r[12], e2y = r[13], e2z = r[14];
@@ -400,7 +452,9 @@ orc_coll orc_mt_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri
     float py = fmaf(dz, e2x, -(dx * e2z));
     float pz = fmaf(dx, e2y, -(dy * e2x));
     float det = fmaf(e1x, px, fmaf(e1y, py, e1z * pz));
-    if (det == 0.0f) return res;
+    float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (det * det < r[11] * d2) return mt_fp64(r, o, p, tri, res);
+    if (det == 0.0f) return res; /* unreachable unless r[11] * d2 == 0 */
     float inv = 1.0f / det;
     float tx = o.x - r[4], ty = o.y - r[5], tz = o.z - r[6];
     float u = fmaf(tx, px, fmaf(ty, py, tz * pz)) * inv;

@@ -129,6 +129,67 @@ def test_injected_physical_mode_bit_exact(ref_inputs, name, scale, boundary, sor
         assert (np.abs(st.pos) <= 1.0).all()
 
 
+def _grazing_sheet(seed, n_side=6, half=0.6):
+    """A flat sheet of 2 n_side^2 triangles in the plane through `origin`
+    spanned by R[:, 0], R[:, 1] (R a seeded rotation)."""
+    rng = np.random.default_rng(seed)
+    a, b, c, d = (lambda v: v / np.linalg.norm(v))(rng.normal(size=4))
+    R = np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                  [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                  [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+    g = np.linspace(-half, half, n_side + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    origin = np.array([0.031, -0.017, 0.0013])
+    verts = (np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1) @ R.T + origin).astype(np.float32)
+    tris = []
+    for i in range(n_side):
+        for j in range(n_side):
+            v = i * (n_side + 1) + j
+            tris += [(v, v + n_side + 1, v + 1), (v + 1, v + n_side + 1, v + n_side + 2)]
+    return verts, np.array(tris, np.uint32), R, origin
+
+
+def test_near_grazing_hits_bit_exact():
+    # every spin's first step crosses a seeded sheet within 3e-5..6e-4 rad of its
+    # plane, so its hit is decided by the FP64 re-test; the steps after it are
+    # ordinary
+    _say("case near-grazing sheet")
+    verts, tris, R, origin = _grazing_sheet(31)
+    m = {"vertices": verts, "trgls": tris, "tri_info": np.zeros((len(tris), 3), np.uint32)}
+    n, iters = 4000, 8
+    std = np.array([0.03, 0.03, 0.01, 0.03], np.float32)
+    w, sub, sim = _pair(m, mode=1, grid="aabb", dims=(16, 16, 16), lo=(-1.2,) * 3,
+                        cell=(2.4 / 16,) * 3, n=n, std=tuple(std))
+    rng = np.random.default_rng(32)
+    xy = rng.uniform(-0.45, 0.45, (n, 2))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    sin_t = 10.0 ** rng.uniform(-4.5, -3.2, n)
+    side = rng.choice([-1.0, 1.0], n)
+    lq, f = rng.uniform(0.6, 1.6, n), rng.uniform(0.1, 0.9, n)
+    pos0 = (np.c_[xy, np.zeros(n)] @ R.T + origin).astype(np.float32)
+    comp0, _, _, _ = w.find_spins(pos0)
+    h = -side * f * std[comp0] * lq * sin_t
+    pos = (np.c_[xy, h] @ R.T + origin).astype(np.float32)
+    comp, fib, _, _ = w.find_spins(pos)
+    cos_t = np.sqrt(1 - sin_t ** 2)
+    d_loc = np.c_[cos_t * np.cos(phi), cos_t * np.sin(phi), side * sin_t] * lq[:, None]
+    q = _inject(n, iters, 33)
+    q[0, :, :3] = (d_loc @ R.T).astype(np.float32)
+    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+    tr_o = O.integrate(w, st, iters, rng=O.RNG_INJECT, inject=q, weight=1.0, trace=True)
+    sim.set_state(pos=pos, comp=comp, fiber=fib)
+    sim.moment_reset()
+    tr_g = sim.integrate_moment_injected(1.0, q, trace=True)
+    g = sim.get_state()
+    mom, _ = sim.get_moment()
+    assert (tr_o[0, :, 2] > 0).sum() > n // 2, "the first steps should mostly hit the sheet"
+    np.testing.assert_array_equal(tr_g, tr_o)
+    np.testing.assert_array_equal(g["pos"], st.pos)
+    np.testing.assert_array_equal(g["mag"], st.mag)
+    np.testing.assert_array_equal(g["comp"], st.comp)
+    np.testing.assert_array_equal(mom, st.moment)
+
+
 @pytest.mark.parametrize("name,scale", CASES)
 def test_labels_match_oracle(ref_inputs, name, scale):
     _say(f"case labels {name} scale={scale}")

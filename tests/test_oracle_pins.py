@@ -192,6 +192,45 @@ def test_mt_agrees_with_exact_brute_force():
     assert agree > 200
 
 
+def test_mt_near_grazing_agrees_with_exact_brute_force():
+    # segments within 1e-6..1e-3 rad of the triangle's plane, crossing it inside
+    # or just outside the triangle: the FP64 re-test must decide them as exact
+    # rational arithmetic does on the stored triangle (v0 and the float edges
+    # e1, e2; at such angles their rounding moves the crossing visibly) and
+    # place the hit to float accuracy
+    rng = np.random.default_rng(23)
+    agree = hits = 0
+    for trial in range(400):
+        tri = rng.uniform(-0.5, 0.5, (3, 3)).astype(np.float32)
+        a, b, c = tri.astype(np.float64)
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        s_, t_ = rng.uniform(-0.3, 1.0, 2)
+        x = a + s_ * (b - a) + t_ * (c - a)
+        w_ = np.cross(n, rng.normal(size=3))
+        w_ /= np.linalg.norm(w_)
+        th = 10.0 ** rng.uniform(-6, -3)
+        d = np.cos(th) * w_ + np.sin(th) * n * rng.choice([-1, 1])
+        L, f = rng.uniform(0.05, 0.5), rng.uniform(0.05, 0.95)
+        o = (x - f * L * d).astype(np.float32)
+        p = (x + (1 - f) * L * d).astype(np.float32)
+        stored = [tri[0].astype(np.float64) + (tri[k] - tri[0]).astype(np.float64) for k in (1, 2)]
+        hit, q, s, t = _exact_seg_tri(o, p, tri[0], *stored)
+        if s is not None and t is not None and min(abs(s), abs(t), abs(1 - s - t)) < 1e-6:
+            continue
+        if s is not None and t is None and min(abs(s), abs(1 - s)) < 1e-6:
+            continue
+        w = _world(tri, [0, 1, 2], grid="aabb", dims=(1, 1, 1), lo=(-1, -1, -1), cell=(2, 2, 2),
+                   eps=0, test=1)
+        r = w.mt_coll(o, p, 0)
+        assert (r.type > 0) == hit, (trial, th)
+        if hit:
+            np.testing.assert_allclose([r.pt.x, r.pt.y, r.pt.z], [float(v) for v in q], atol=2e-7)
+            hits += 1
+        agree += 1
+    assert agree > 350 and hits > 100
+
+
 @pytest.mark.parametrize("test", [0, 1])
 def test_reflection_householder_invariants(test):
     rng = np.random.default_rng(3)
