@@ -155,6 +155,30 @@ def served_bench(n_requests, plain_requests):
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 matrix, FLOP/s
 
 
+def substrate_prep(env):
+    """Time to a resident substrate: the host build (multi-threaded C++) and
+    the device build of the same records, grid and clearance, each from the
+    mesh arrays in host memory; the grids and clearances are compared."""
+    g = env["grid"]
+    kw = dict(grid="aabb", dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+    mesh = (env["vertices"], env["triangles"], env["tri_info"])
+    D.Substrate(*mesh, device=True, **kw).close()  # first use of the build kernels
+    t0 = time.perf_counter()
+    h = D.Substrate(*mesh, **kw)
+    t1 = time.perf_counter()
+    d = D.Substrate(*mesh, device=True, **kw)
+    t2 = time.perf_counter()
+    sa, ta, ma = h.grid()
+    sb, tb, mb = d.grid()
+    same = bool(np.array_equal(sa, sb) and np.array_equal(ta, tb) and ma == mb
+                and np.array_equal(h.clearance(), d.clearance()))
+    h.close()
+    d.close()
+    return {"triangles": len(env["triangles"]), "cells": int(np.prod(g["dims"])),
+            "entries": int(len(ta)), "host_s": round(t1 - t0, 4), "device_s": round(t2 - t1, 4),
+            "identical": same}
+
+
 def waveform_bench(n_spins, n_wave, sample):
     """Waveform-phase path: n_spins spins on the small substrate, n_wave random
     piecewise-constant waveforms (|G| <= 80 mT/m, a new vector every 10 us) over
@@ -169,7 +193,7 @@ def waveform_bench(n_spins, n_wave, sample):
     env = S.build(cfg, cell_um=0.2, axons=S.pack(cfg))
     g = env["grid"]
     sub = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
-                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"], device=True)
     pc = np.float32(2 * np.pi * GAMMA * cfg.dt)
     grads = PH.random_waveforms(n_wave, steps, seed=17) * np.float32(1e-9 * env["space_scale"])
     log(f"waveforms: substrate and {n_wave} waveforms in {time.perf_counter() - t0:.1f} s")
@@ -327,6 +351,10 @@ def main():
 
     env = S.build(cfg, cell_um=cell_um, axons=axons)
     g = env["grid"]
+    prep = substrate_prep(env)
+    log(f"substrate prep ({prep['triangles']} triangles, {prep['cells']} cells): host build "
+        f"{prep['host_s']:.3f} s, device build {prep['device_s']:.3f} s, identical "
+        f"{prep['identical']}")
     sub, sim = AT.make_sim(env, n, sort_interval=sort_interval)
     _, _, maxc = sub.grid()
     t0 = time.perf_counter()
@@ -414,6 +442,7 @@ def main():
                    "max_tris_per_cell": maxc, "parallelism": "none (1 GPU)"},
         "reference_equivalent_value": value * len(prot),
         "autotune": tuned,
+        "substrate_prep": prep,
         "roofline": roof,
         "cpu_baseline": cpu,
         "served": served,

@@ -18,11 +18,12 @@ from . import lib as D
 from . import substrate as S
 
 
-def make_sim(env, n, *, sort_interval, steps_per_launch=0, seed=None):
+def make_sim(env, n, *, sort_interval, steps_per_launch=0, seed=None, device_build=True):
     g = env["grid"]
     cfg = env["cfg"]
     sub = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
-                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+                      dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"],
+                      device=device_build)
     sim = D.Simulation(sub, n, dt=cfg.dt, std_dev=env["std_dev"], t2=env["t2"],
                        permeability=env["permeability"], mode=D.MODE_PHYSICAL,
                        boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX,

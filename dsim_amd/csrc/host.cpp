@@ -194,7 +194,8 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
 // triangle refine that with the smallest box-to-box distance between the
 // cell and the bounding boxes of the triangles listed in the rings around it
 // (growing Chebyshev radius, stopping once a ring is farther than the best
-// found).  Rounded down to float; 1e30 when the grid holds no triangle.
+// found).  Rounded down to float; 1e30 when the grid holds no triangle.  The
+// device build (grid.hip) computes the same values.
 void compute_clearance(const uint32_t dims[3], const float lo[3], const float cell[3],
                        const std::vector<uint32_t> &start, const std::vector<uint32_t> &tri,
                        const std::vector<float4> &box, std::vector<float> &out) {
@@ -239,9 +240,14 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
                         b0[k] = (double)lo[k] + (double)idx[k] * cell[k];
                         b1[k] = b0[k] + cell[k];
                     }
-                    best = 1e30;
+                    // squared distances; the float result is the largest float
+                    // whose square is at most the minimum
+                    double best2 = 1e60;
                     for (int64_t r = ring[c];; ++r) {
-                        if (r > 0 && (double)(r - 1) * hmin >= best) break;
+                        if (r > 0) {
+                            const double q = (double)(r - 1) * hmin;
+                            if (q * q >= best2) break;
+                        }
                         if (r > D[0] && r > D[1] && r > D[2]) break;
                         for (int64_t z = std::max<int64_t>(0, iz - r); z <= std::min(D[2] - 1, iz + r); ++z)
                             for (int64_t y = std::max<int64_t>(0, iy - r); y <= std::min(D[1] - 1, iy + r); ++y)
@@ -256,10 +262,12 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
                                             double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
                                             d2 += g * g;
                                         }
-                                        best = std::min(best, std::sqrt(d2));
+                                        best2 = std::min(best2, d2);
                                     }
                                 }
                     }
+                    out[c] = dsim_host::float_sqrt_floor(best2);
+                    continue;
                 }
                 float f = (float)best;
                 if ((double)f > best) f = std::nextafter(f, 0.0f);
@@ -276,6 +284,23 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
 // Substrate
 // ---------------------------------------------------------------------------
 
+int dsim_host::check_mesh(const dsim_mesh *mesh) {
+    if (mesh->n_triangles && (!mesh->vertices || !mesh->triangles || !mesh->tri_info))
+        return fail(DSIM_ERR_ARG, "mesh arrays missing");
+    const uint32_t nT = mesh->n_triangles, nV = mesh->n_vertices;
+    for (uint64_t k = 0; k < (uint64_t)nT * 3; ++k)
+        if (mesh->triangles[k] >= nV)
+            return fail(DSIM_ERR_ARG, "triangle %llu references vertex %u of %u",
+                        (unsigned long long)(k / 3), mesh->triangles[k], nV);
+    for (uint32_t t = 0; t < nT; ++t) {
+        if (mesh->tri_info[t * 3 + 1] > 2) return fail(DSIM_ERR_ARG, "triangle %u membrane > 2", t);
+        if (mesh->tri_info[t * 3] >= DSIM_NO_FIBER)
+            return fail(DSIM_ERR_ARG, "triangle %u fibre index too large", t);
+        if (mesh->tri_info[t * 3 + 2] >= nV) return fail(DSIM_ERR_ARG, "triangle %u P1 bad", t);
+    }
+    return DSIM_OK;
+}
+
 extern "C" const char *dsim_last_error(void) { return dsim_host::g_err.c_str(); }
 extern "C" int dsim_abi_version(void) { return DSIM_ABI_VERSION; }
 
@@ -291,19 +316,8 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
                                      dsim_substrate **out) {
     if (!mesh || !grid || !out) return fail(DSIM_ERR_ARG, "NULL argument");
     *out = nullptr;
-    if (mesh->n_triangles && (!mesh->vertices || !mesh->triangles || !mesh->tri_info))
-        return fail(DSIM_ERR_ARG, "mesh arrays missing");
+    if (int rc = dsim_host::check_mesh(mesh)) return rc;
     const uint32_t nT = mesh->n_triangles, nV = mesh->n_vertices;
-    for (uint64_t k = 0; k < (uint64_t)nT * 3; ++k)
-        if (mesh->triangles[k] >= nV)
-            return fail(DSIM_ERR_ARG, "triangle %llu references vertex %u of %u",
-                        (unsigned long long)(k / 3), mesh->triangles[k], nV);
-    for (uint32_t t = 0; t < nT; ++t) {
-        if (mesh->tri_info[t * 3 + 1] > 2) return fail(DSIM_ERR_ARG, "triangle %u membrane > 2", t);
-        if (mesh->tri_info[t * 3] >= DSIM_NO_FIBER)
-            return fail(DSIM_ERR_ARG, "triangle %u fibre index too large", t);
-        if (mesh->tri_info[t * 3 + 2] >= nV) return fail(DSIM_ERR_ARG, "triangle %u P1 bad", t);
-    }
     int rc = DSIM_OK;
     int ndev = 0;
     const bool have_device = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
@@ -493,7 +507,7 @@ extern "C" int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cel
                                         uint32_t *n_entries, uint32_t *max_per_cell) {
     if (!s) return fail(DSIM_ERR_ARG, "NULL substrate");
     if (n_cells) *n_cells = s->n_cells;
-    if (n_entries) *n_entries = (uint32_t)s->cell_tri.size();
+    if (n_entries) *n_entries = s->device_built ? s->n_entries : (uint32_t)s->cell_tri.size();
     if (max_per_cell) *max_per_cell = s->max_per_cell;
     return DSIM_OK;
 }
@@ -501,6 +515,15 @@ extern "C" int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cel
 extern "C" int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_start,
                                        uint32_t *cell_tri) {
     if (!s) return fail(DSIM_ERR_ARG, "NULL substrate");
+    if (s->device_built) {
+        if (cell_start)
+            HIP_TRY(hipMemcpy(cell_start, s->d_cell_start.p, ((size_t)s->n_cells + 1) * 4,
+                              hipMemcpyDeviceToHost));
+        if (cell_tri && s->n_entries)
+            HIP_TRY(hipMemcpy(cell_tri, s->d_cell_tri.p, (size_t)s->n_entries * 4,
+                              hipMemcpyDeviceToHost));
+        return DSIM_OK;
+    }
     if (cell_start) memcpy(cell_start, s->cell_start.data(), s->cell_start.size() * 4);
     if (cell_tri && !s->cell_tri.empty()) memcpy(cell_tri, s->cell_tri.data(), s->cell_tri.size() * 4);
     return DSIM_OK;
@@ -508,6 +531,11 @@ extern "C" int dsim_substrate_get_grid(const dsim_substrate *s, uint32_t *cell_s
 
 extern "C" int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance) {
     if (!s || !clearance) return fail(DSIM_ERR_ARG, "NULL argument");
+    if (s->device_built && s->d_clearance.p) {
+        HIP_TRY(hipMemcpy(clearance, s->d_clearance.p, (size_t)s->n_cells * 4,
+                          hipMemcpyDeviceToHost));
+        return DSIM_OK;
+    }
     if (s->clearance.empty()) return fail(DSIM_ERR_STATE, "the grid has no clearance table");
     memcpy(clearance, s->clearance.data(), s->clearance.size() * 4);
     return DSIM_OK;
@@ -515,6 +543,11 @@ extern "C" int dsim_substrate_get_clearance(const dsim_substrate *s, float *clea
 
 extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers) {
     if (!s || !helpers) return fail(DSIM_ERR_ARG, "NULL argument");
+    if (s->device_built) {
+        if (s->n_tri)
+            HIP_TRY(hipMemcpy(helpers, s->d_helpers.p, (size_t)s->n_tri * 48, hipMemcpyDeviceToHost));
+        return DSIM_OK;
+    }
     if (!s->helpers.empty()) memcpy(helpers, s->helpers.data(), s->helpers.size() * 4);
     return DSIM_OK;
 }
@@ -523,7 +556,8 @@ extern "C" int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(DSIM_ERR_ARG, "NULL argument");
     *bytes = (uint64_t)s->tri_ref.n * 16 + s->tri_mt.n * 16 + s->tri_aabb.n * 16 + s->cell_plane.n * 16 +
              s->cell_mt.n * 16 +
-             s->d_cell_start.n * 4 + s->d_cell_tri.n * 4 + s->fiber_myelin.n;
+             s->d_cell_start.n * 4 + s->d_cell_tri.n * 4 + s->fiber_myelin.n + s->d_clearance.n * 4 +
+             s->d_helpers.n * 4;
     return DSIM_OK;
 }
 

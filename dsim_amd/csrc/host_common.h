@@ -16,6 +16,24 @@ namespace dsim_host {
 // Records a message for dsim_last_error() and returns code.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 
+// Checks the mesh arrays of dsim_substrate_create (indices, membranes, fibres).
+int check_mesh(const dsim_mesh *mesh);
+
+// The largest float whose square is at most x >= 0 (the float floor of the
+// square root, whatever the rounding of the double square root: the square of
+// a float is exact in double).
+inline float float_sqrt_floor(double x) {
+    if (!(x >= 0.0)) return 0.0f;
+    float f = (float)__builtin_sqrt(x);
+    while (f > 0.0f && (double)f * (double)f > x) f = __builtin_nextafterf(f, 0.0f);
+    for (;;) {
+        const float g = __builtin_nextafterf(f, 3.0e38f);
+        if ((double)g * (double)g > x) break;
+        f = g;
+    }
+    return f;
+}
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -70,13 +88,17 @@ struct dsim_substrate {
     std::vector<float> clearance;  // per cell: lower bound of the distance to any triangle
     dsim_host::DevBuf<float4> tri_ref, tri_mt, tri_aabb, cell_plane, cell_mt;
     dsim_host::DevBuf<uint32_t> d_cell_start, d_cell_tri;
-    dsim_host::DevBuf<float> d_clearance;
+    dsim_host::DevBuf<float> d_clearance, d_helpers;
     dsim_host::DevBuf<uint8_t> fiber_myelin;
     int refs = 0;
     bool on_device = false;
+    // built by dsim_substrate_create_device: the grid, helpers and clearance
+    // live on the device only (the host vectors stay empty)
+    bool device_built = false;
+    uint32_t n_entries = 0;
     ~dsim_substrate() {
         tri_ref.release(); tri_mt.release(); tri_aabb.release(); cell_plane.release(); cell_mt.release();
-        d_clearance.release();
+        d_clearance.release(); d_helpers.release();
         d_cell_start.release(); d_cell_tri.release(); fiber_myelin.release();
     }
 };

@@ -91,6 +91,8 @@ def lib():
         sigs = {
             "dsim_device_count": [C.POINTER(C.c_int)],
             "dsim_substrate_create": [C.POINTER(Mesh), C.POINTER(GridParams), C.POINTER(_vp)],
+            "dsim_substrate_create_device": [C.POINTER(Mesh), C.POINTER(GridParams),
+                                             C.POINTER(_vp)],
             "dsim_substrate_destroy": [_vp],
             "dsim_substrate_grid_info": [_vp, _up, _up, _up],
             "dsim_substrate_get_grid": [_vp, _up, _up],
@@ -141,10 +143,12 @@ def _p(a, t):
 
 
 class Substrate:
-    """Triangle mesh + uniform grid resident on the device."""
+    """Triangle mesh + uniform grid resident on the device.  device=True builds
+    the records, grid and clearance on the GPU (dsim_substrate_create_device);
+    the arrays equal the host build's."""
 
     def __init__(self, vertices, triangles, tri_info, *, grid="reference", n_cubes=50, dims=None,
-                 lo=None, cell=None, eps=0.0):
+                 lo=None, cell=None, eps=0.0, device=False):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.triangles = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
         self.tri_info = np.ascontiguousarray(tri_info, np.uint32).reshape(-1, 3)
@@ -161,7 +165,8 @@ class Substrate:
             g.cell = (C.c_float * 3)(*[float(x) for x in cell])
             g.eps = float(eps)
         h = _vp()
-        check(lib().dsim_substrate_create(C.byref(m), C.byref(g), C.byref(h)))
+        create = lib().dsim_substrate_create_device if device else lib().dsim_substrate_create
+        check(create(C.byref(m), C.byref(g), C.byref(h)))
         self.h = h
 
     def close(self):

@@ -57,7 +57,8 @@ def make_substrates(sc: StreamConfig, log=None):
         env = S.build(cfg, cell_um=sc.cell_sigma * sigma_um, axons=S.pack(cfg))
         g = env["grid"]
         env["sub"] = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
-                                 dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"])
+                                 dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"],
+                                 device=True)
         if log:
             log(f"served substrate {k}: {n_ax} axons, mean radius {r:.2f} um, "
                 f"{len(env['triangles'])} triangles, grid {g['dims']} "

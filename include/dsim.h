@@ -119,6 +119,14 @@ int dsim_device_count(int *n);
  * ascending triangle order. */
 int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_params *grid,
                           dsim_substrate **out);
+/* The same substrate built on the device: triangle records, the grid (the
+ * reference's populateCubes membership, spinSystem.cpp:1094-1352, or the AABB
+ * membership), its cell-major copies and the AABB clearance are computed by
+ * GPU kernels with the host build's arithmetic, so every array equals
+ * dsim_substrate_create's.  Needs a HIP device; the host keeps no copy of the
+ * grid (the getters below read it back). */
+int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_grid_params *grid,
+                                 dsim_substrate **out);
 int dsim_substrate_destroy(dsim_substrate *s);
 /* Grid sizes: cells, list entries, and the reference's m_maxTrianglesPerCube. */
 int dsim_substrate_grid_info(const dsim_substrate *s, uint32_t *n_cells, uint32_t *n_entries,
