@@ -162,20 +162,19 @@ __device__ __forceinline__ void test_ref(const float4 *__restrict__ R, uint32_t 
 
 // A segment within about 1e-3 rad of a triangle's plane, det^2 < 1e-6
 // |e1 x e2|^2 |d|^2, is decided in FP64; the triangle record carries
-// 1e-6 |e1 x e2|^2 in e1.w.  The plane prefilter keeps every segment whose end
-// points' signed distances s0, s1 (normalised units) have s0 s1 <= kCrossTol:
-// all true crossings, and same-side segments whose rounded distances could
-// belong to a crossing -- for a crossing within 1e-3 rad, one of s0, s1 is
-// within the distance's rounding error (below 4e-7 for coordinates within
-// [-1, 1]) of zero and the other below 1e-3 times the step length.
-constexpr float kCrossTol = 1e-9f;
+// 1e-6 |e1 x e2|^2 in e1.w.
 
-// Plane-side prefilter: can the segment o-p cross the plane (n, n.v0)?
+// Plane-side prefilter: do the end points lie on different sides of the plane
+// (n, n.v0) as this expression evaluates it?  A point's side is a fixed
+// function of its coordinates, so a spin changes side only across a segment
+// that passes here.  (Admitting same-side segments with an end point near the
+// plane lets a reflection point on the edge shared with a neighbouring
+// triangle hit that neighbour at t ~ 0, and back, without end.)
 __device__ __forceinline__ bool crosses_plane(const float4 pl, float ox, float oy, float oz,
                                               float px, float py, float pz) {
     float s0 = fmaf(pl.x, ox, fmaf(pl.y, oy, fmaf(pl.z, oz, -pl.w)));
     float s1 = fmaf(pl.x, px, fmaf(pl.y, py, fmaf(pl.z, pz, -pl.w)));
-    return s0 * s1 <= kCrossTol;
+    return (s0 > 0.0f) != (s1 > 0.0f);
 }
 
 __device__ __forceinline__ float seg_len2(float ox, float oy, float oz, float px, float py,

@@ -141,11 +141,9 @@ void orc_tri_helpers(const float *V, const uint32_t *T, uint32_t n_tri, float *H
     }
 }
 
-/* Build-defined tolerances of the production test (orc_mt_coll_detect): the
- * plane prefilter keeps segments whose end points' signed plane distances
- * have s0 s1 <= ORC_CROSS_TOL; segments within about 1e-3 rad of the plane
- * (det^2 < ORC_GRAZE_SQ |e1 x e2|^2 |d|^2) are decided in FP64. */
-#define ORC_CROSS_TOL 1e-9f
+/* Build-defined: segments within about 1e-3 rad of a triangle's plane
+ * (det^2 < ORC_GRAZE_SQ |e1 x e2|^2 |d|^2) are decided in FP64
+ * (orc_mt_coll_detect). */
 #define ORC_GRAZE_SQ 1e-6f
 
 /* Build-defined 64-byte Moller-Trumbore record: plane (n, n.v0), v0, e1=v1-v0
@@ -429,8 +427,8 @@ static orc_coll mt_fp64(const float *r, orc_f3 o, orc_f3 p, uint32_t tri, orc_co
     return res;
 }
 
-/* Build-defined production test: plane-side prefilter (s0 s1 <= ORC_CROSS_TOL),
- * then Moller-Trumbore
+/* Build-defined production test: plane-side prefilter (end points on different
+ * sides as the float expression evaluates them), then Moller-Trumbore
  * ("Fast, minimum storage ray/triangle intersection", JGT 1997) restricted to
  * the open segment 0<t<1, in FP64 when the segment is within about 1e-3 rad
  * of the plane (det^2 < ORC_GRAZE_SQ |e1 x e2|^2 |d|^2).  dist_sq carries t. */
@@ -444,7 +442,7 @@ orc_coll orc_mt_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri
     const float *r = w->tri_mt + (size_t)tri * 16;
     float s0 = fmaf(r[0], o.x, fmaf(r[1], o.y, fmaf(r[2], o.z, -r[3])));
     float s1 = fmaf(r[0], p.x, fmaf(r[1], p.y, fmaf(r[2], p.z, -r[3])));
-    if (s0 * s1 > ORC_CROSS_TOL) return res;
+    if ((s0 > 0.0f) == (s1 > 0.0f)) return res;
     float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
     float e1x = r[8], e1y = r[9], e1z = r[10];
     float e2x = r[12], e2y = r[13], e2z = r[14];
