@@ -97,11 +97,9 @@ struct WalkTable {
     WalkJob job[kMaxJobs];
 };
 
-// Launchers (kernels.hip).  Return hipError_t of the launch.  The walk kernel
-// (physical mode) reads its parameters from dev_p, a device copy of p that
-// the caller has written on stream s.
-hipError_t launch_step(const StepParams &p, const StepParams *dev_p, uint32_t iters, int mode,
-                       int periodic, int rng, bool trace, bool moment, hipStream_t s);
+// Launchers (kernels.hip).  Return hipError_t of the launch.
+hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
+                       bool trace, bool moment, hipStream_t s);
 hipError_t launch_signal(uint32_t n, const float *mag, const float *phase, const uint32_t *label,
                          double *partial, uint32_t n_blocks, double *out15, hipStream_t s);
 hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, const float *x0,

@@ -1033,14 +1033,11 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     }
 }
 
-// The parameters come from device memory rather than the kernel arguments: the
-// compiler then loads them where they are used instead of holding all of them
-// in scalar registers (which spill into vector registers).
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
-    const StepParams *__restrict__ PP, uint32_t iters) {
+    StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[4];
-    walk_body<PERIODIC, RNG, TRACE, MOMENT>(*PP, xcd_block(), iters, scratch[threadIdx.x >> 6]);
+    walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(), iters, scratch[threadIdx.x >> 6]);
 }
 
 // Batched walk for the server: the blocks of one launch are shared out over
@@ -1074,45 +1071,44 @@ hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool 
 }
 
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
-static void launch_one(const StepParams &p, const StepParams *dev_p, uint32_t iters, hipStream_t s) {
+static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     dim3 grid((p.n + 255) / 256), block(256);
     if constexpr (MODE == 0) {
         hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
     } else {
-        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, dev_p,
-                           iters);
+        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
     }
 }
 
 template <int MODE, bool PERIODIC, int RNG>
-static hipError_t launch3(const StepParams &p, const StepParams *dev_p, uint32_t iters, bool trace, bool moment,
+static hipError_t launch3(const StepParams &p, uint32_t iters, bool trace, bool moment,
                           hipStream_t s) {
     if (trace) {
-        if (moment) launch_one<MODE, PERIODIC, RNG, true, true>(p, dev_p, iters, s);
-        else launch_one<MODE, PERIODIC, RNG, true, false>(p, dev_p, iters, s);
+        if (moment) launch_one<MODE, PERIODIC, RNG, true, true>(p, iters, s);
+        else launch_one<MODE, PERIODIC, RNG, true, false>(p, iters, s);
     } else {
-        if (moment) launch_one<MODE, PERIODIC, RNG, false, true>(p, dev_p, iters, s);
-        else launch_one<MODE, PERIODIC, RNG, false, false>(p, dev_p, iters, s);
+        if (moment) launch_one<MODE, PERIODIC, RNG, false, true>(p, iters, s);
+        else launch_one<MODE, PERIODIC, RNG, false, false>(p, iters, s);
     }
     return hipGetLastError();
 }
 
 template <int MODE, bool PERIODIC>
-static hipError_t launch2(const StepParams &p, const StepParams *dev_p, uint32_t iters, int rng, bool trace, bool moment,
+static hipError_t launch2(const StepParams &p, uint32_t iters, int rng, bool trace, bool moment,
                           hipStream_t s) {
     switch (rng) {
-        case kRngInject: return launch3<MODE, PERIODIC, kRngInject>(p, dev_p, iters, trace, moment, s);
-        case kRngMwc: return launch3<MODE, PERIODIC, kRngMwc>(p, dev_p, iters, trace, moment, s);
-        default: return launch3<MODE, PERIODIC, kRngPhilox>(p, dev_p, iters, trace, moment, s);
+        case kRngInject: return launch3<MODE, PERIODIC, kRngInject>(p, iters, trace, moment, s);
+        case kRngMwc: return launch3<MODE, PERIODIC, kRngMwc>(p, iters, trace, moment, s);
+        default: return launch3<MODE, PERIODIC, kRngPhilox>(p, iters, trace, moment, s);
     }
 }
 
-hipError_t launch_step(const StepParams &p, const StepParams *dev_p, uint32_t iters, int mode,
-                       int periodic, int rng, bool trace, bool moment, hipStream_t s) {
+hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
+                       bool trace, bool moment, hipStream_t s) {
     if (p.n == 0 || iters == 0) return hipSuccess;
-    if (mode == 0) return launch2<0, false>(p, dev_p, iters, rng, trace, moment, s);
-    if (periodic) return launch2<1, true>(p, dev_p, iters, rng, trace, moment, s);
-    return launch2<1, false>(p, dev_p, iters, rng, trace, moment, s);
+    if (mode == 0) return launch2<0, false>(p, iters, rng, trace, moment, s);
+    if (periodic) return launch2<1, true>(p, iters, rng, trace, moment, s);
+    return launch2<1, false>(p, iters, rng, trace, moment, s);
 }
 
 // ------------------------------------------------------------------------ //

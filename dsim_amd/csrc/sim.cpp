@@ -41,7 +41,6 @@ struct dsim_sim {
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> state, alt;  // F_COUNT * n words each (alt only when sorting)
     DevBuf<uint32_t> seed, flags, tries;
-    DevBuf<dsim::StepParams> params;  // device copy of the walk launch's parameters
     DevBuf<double> partial, out;
     DevBuf<float> grads;
     DevBuf<float4> inject;
@@ -108,7 +107,6 @@ extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32
     const size_t m = std::max<uint32_t>(n, 1);
     int rc;
     if ((rc = S->state.alloc(m * F_COUNT)) || (rc = S->flags.alloc(4)) ||
-        (rc = S->params.alloc(1)) ||
         (rc = S->partial.alloc((size_t)dsim::kSignalBlocks * 15)) || (rc = S->out.alloc(15)) ||
         (rc = S->tries.alloc(m)) ||
         (p->rng == DSIM_RNG_MWC && (rc = S->seed.alloc(2 * m)))) {
@@ -393,14 +391,11 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
         // each step launch is bracketed by its own events: the timing covers
         // the step kernel only, not the reorders between launches
-        // the walk kernel reads its parameters from device memory; the copy is
-        // ordered on the stream after the previous launch
-        HIP_TRY(hipMemcpyAsync(s->params.p, &P, sizeof(P), hipMemcpyHostToDevice, s->stream));
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, s->stream));
-        hipError_t e = dsim::launch_step(P, s->params.p, k, (int)s->prm.mode,
+        hipError_t e = dsim::launch_step(P, k, (int)s->prm.mode,
                                          s->prm.boundary == DSIM_BOUNDARY_PERIODIC, rng,
                                          trace != nullptr, moment, s->stream);
         if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
