@@ -153,6 +153,7 @@ def served_bench(n_requests, plain_requests):
 
 
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 matrix, FLOP/s
+PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
 
 
 def substrate_prep(env):
@@ -217,6 +218,8 @@ def waveform_bench(n_spins, n_wave, sample):
     wf.close()
     sim.close()
     flops = 3 * 2.0 * n_spins * n_wave * 3 * steps   # hi*hi + hi*lo + lo*hi
+    n_chunks = -(-steps // chunk)
+    gemm_bytes = n_chunks * (16.0 * n_spins * n_wave) + 4.0 * n_spins * 3 * steps
     log(f"waveforms: {n_spins} spins x {steps} steps x {n_wave} waveforms in {wall:.2f} s "
         f"(walk {walk_ms / 1e3:.2f} s, phase GEMM {gemm_ms / 1e3:.2f} s = "
         f"{flops / (gemm_ms / 1e3) / 1e12:.0f} TFLOP/s); b=0-free signal range "
@@ -253,9 +256,16 @@ def waveform_bench(n_spins, n_wave, sample):
             "gemm_k": 3 * chunk, "substrate": cfg.name, "seconds": wall,
             "spin_steps_per_s": n_spins * steps / wall, "walk_s": walk_ms / 1e3,
             "phase_gemm_s": gemm_ms / 1e3,
-            "phase_gemm_roofline": {"bound": "mfma", "achieved": flops / (gemm_ms / 1e3) / 1e12,
-                                    "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                                    "frac": flops / (gemm_ms / 1e3) / PEAK_BF16},
+            # at 64 steps per chunk the GEMM is bound by the read-modify-write of
+            # the (phase, compensation) pairs, 8 B per spin and waveform, plus
+            # one read of the chunk's trajectory (4 B per spin and k)
+            "phase_gemm_roofline": {"bound": "hbm", "achieved": gemm_bytes / (gemm_ms / 1e3) / 1e9,
+                                    "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                    "frac": gemm_bytes / (gemm_ms / 1e3) / PEAK_HBM,
+                                    "traffic": committed_gemm_traffic(n_spins, n_wave, chunk),
+                                    "traffic_unit": "bytes per full chunk",
+                                    "mfma_tflops": flops / (gemm_ms / 1e3) / 1e12,
+                                    "mfma_frac_of_bf16_peak": flops / (gemm_ms / 1e3) / PEAK_BF16},
             "fp64_check": {"spins": sample, "phase_err_max_rad": float(dphi.max()),
                            "phase_err_median_rad": float(np.median(dphi)),
                            "signal_err_max": float(dsig.max()),
@@ -278,6 +288,25 @@ def committed_traffic(cell_um, sort_interval, n):
             f = dict(kv.split("=") for kv in ln.split()[1:])
             if (abs(float(f["cell_um"]) - cell_um) < 1e-9 and int(f["sort"]) == sort_interval
                     and int(f["spins"]) == n):
+                return int(f["bytes"])
+    return None
+
+
+GEMM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_phase_gemm_small-16axon-4e6x1024_hbm_traffic.md")
+
+
+def committed_gemm_traffic(n_spins, n_wave, chunk):
+    """HBM bytes per full-chunk launch of k_phase_gemm from the committed
+    FETCH_SIZE / WRITE_SIZE summary for the matching sizes; else None."""
+    try:
+        lines = open(GEMM_TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        if ln.startswith("traffic_bytes_per_chunk"):
+            f = dict(kv.split("=") for kv in ln.split()[1:])
+            if (int(f["spins"]) == n_spins and int(f["waveforms"]) == n_wave
+                    and int(f["chunk"]) == chunk):
                 return int(f["bytes"])
     return None
 
