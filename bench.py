@@ -98,7 +98,7 @@ def cpu_baseline(env, target_s, threads):
                 n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
 
 
-def served_bench(n_requests, plain_requests):
+def served_bench(n_requests, plain_requests, round_steps=256):
     """The served form against the plain one on a stream of requests over
     resident substrates: capacity (every request queued at once) in
     simulations per hour, the same requests one at a time through the plain
@@ -112,7 +112,7 @@ def served_bench(n_requests, plain_requests):
     envs = SVD.make_substrates(sc, log)
     log(f"served: {sc.n_substrates} resident substrates in {time.perf_counter() - t0:.1f} s")
     reqs = SVD.make_requests(sc, envs)
-    out, mk = SVD.run_served(envs, reqs, realtime=False)
+    out, mk = SVD.run_served(envs, reqs, realtime=False, round_steps=round_steps)
     cap = len(out) / mk
     log(f"served: {len(out)} requests ({SVD.work(reqs):.3e} spin-steps) in {mk:.2f} s")
     sub = reqs[:plain_requests]
@@ -125,7 +125,8 @@ def served_bench(n_requests, plain_requests):
     streams = []
     for load in (0.4, 0.7):
         rate = load * cap
-        live, _ = SVD.run_served(envs, SVD.make_requests(sc, envs, rate=rate), realtime=True)
+        live, _ = SVD.run_served(envs, SVD.make_requests(sc, envs, rate=rate), realtime=True,
+                                 round_steps=round_steps)
         lat = np.array([o["latency"] for o in live])
         vs = lat[:len(solo)] / solo
         streams.append({"load": load, "rate_per_s": rate,
@@ -138,7 +139,7 @@ def served_bench(n_requests, plain_requests):
             f"{np.percentile(vs, 50):.2f}x, p99 {np.percentile(vs, 99):.2f}x")
     for e in envs:
         e["sub"].close()
-    return {"requests": len(reqs), "substrates": sc.n_substrates,
+    return {"requests": len(reqs), "substrates": sc.n_substrates, "round_steps": round_steps,
             "sims_per_hour": cap * 3600, "plain_sims_per_hour": len(sub) / t_plain * 3600,
             "speedup_vs_plain": cap / (len(sub) / t_plain),
             "served_spin_steps_per_s": SVD.work(reqs) / mk,
@@ -341,6 +342,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=120,
                     help="requests in the served-form comparison (0 skips it)")
     ap.add_argument("--plain-requests", type=int, default=40)
+    ap.add_argument("--round-steps", type=int, default=256,
+                    help="steps per batched round of the served form")
     ap.add_argument("--waveform-spins", type=int, default=4_000_000,
                     help="spins of the waveform-phase measurement (0 skips it)")
     ap.add_argument("--waveforms", type=int, default=1024)
@@ -445,7 +448,8 @@ def main():
     if args.served_requests:
         sim.close()
         sub.close()
-        served = served_bench(args.served_requests, min(args.plain_requests, args.served_requests))
+        served = served_bench(args.served_requests, min(args.plain_requests, args.served_requests),
+                              args.round_steps)
     waves = None
     if args.waveform_spins:
         waves = waveform_bench(args.waveform_spins, args.waveforms, 8192)
