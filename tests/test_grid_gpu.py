@@ -96,3 +96,19 @@ def test_generated_substrate_and_walk_equal_host_build():
         sim.close()
     for k in ("pos", "mag", "comp", "fiber"):
         np.testing.assert_array_equal(states[1][k], states[0][k])
+
+
+def test_empty_mesh_and_partial_grid_equal_host_build(ref_inputs):
+    _say("case device grid edge cases")
+    # no triangles: empty lists, clearance 1e30 everywhere
+    m = {"vertices": np.zeros((3, 3), np.float32), "trgls": np.zeros((0, 3), np.uint32),
+         "tri_info": np.zeros((0, 3), np.uint32)}
+    a, b = _both(m, grid="aabb", dims=(5, 4, 3), lo=(-1.0,) * 3, cell=(0.4, 0.5, 0.6), eps=0.0)
+    assert _same(a, b, clearance=True) == 0
+    assert (b.clearance() == np.float32(1e30)).all()
+    # a grid over part of a mesh: triangles outside it are dropped, those
+    # straddling its faces are kept
+    m = O.parse_mesh(os.path.join(ref_inputs, "sim.fibers_1_r004_f09"), 6.0)
+    a, b = _both(m, grid="aabb", dims=(17, 11, 7), lo=(-0.3, -0.2, -0.5), cell=(0.03, 0.04, 0.05),
+                 eps=2e-5)
+    assert _same(a, b, clearance=True) > 0
