@@ -746,7 +746,8 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
 // lane with `search` set, either finds the nearest hit of the segment o-p
 // (excluding `excluded`): returns true with key = hit_key(t, triangle) (or ~0
 // when there is none) and the hit point and type in `hit`; or returns false
-// when the wave's cell list is full this round (the lane searches again).
+// when the wave's cell list is full in this search round (the lane searches
+// again).
 __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L, uint32_t lane,
                                             bool search, float ox, float oy, float oz, float px,
                                             float py, float pz, uint32_t excluded,
