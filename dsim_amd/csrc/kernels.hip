@@ -746,12 +746,16 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
 // lane with `search` set, either finds the nearest hit of the segment o-p
 // (excluding `excluded`): returns true with key = hit_key(t, triangle) (or ~0
 // when there is none) and the hit point and type in `hit`; or returns false
-// when the wave's cell list is full in this search round (the lane searches
-// again).
+// when the wave's cell list is full in this search round, or when the lane's
+// cell box is longer than the list and cells remain (the lane searches again).
+// A box of more than kWaveSegs cells is searched kWaveSegs cells per round, in
+// x-fastest order from cell `slice` (0 for a new segment; advanced here), its
+// nearest hit carried in best[]/hit[] from round to round.
 __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L, uint32_t lane,
                                             bool search, float ox, float oy, float oz, float px,
                                             float py, float pz, uint32_t excluded,
-                                            unsigned long long &key, float4 &hit) {
+                                            uint32_t &slice, unsigned long long &key,
+                                            float4 &hit) {
     uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0, ncell = 0;
     if (search) {
         uint32_t sx, sy, sz, tx, ty, tz;
@@ -762,22 +766,28 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
         z0 = min(sz, tz); z1 = max(sz, tz);
         ncell = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
     }
-    // a lane whose box alone overflows the list searches on its own; one that
-    // does not fit behind the lanes before it waits for the next round
-    const bool alone = search && ncell > kWaveSegs;
-    const uint32_t nreq = (search && !alone) ? ncell : 0u;
+    // a lane lists at most kWaveSegs cells of its box per round; one that does
+    // not fit behind the lanes before it waits for the next round (the first
+    // lane always fits)
+    const uint32_t nreq = search ? min(ncell - slice, kWaveSegs) : 0u;
     const uint32_t send = wave_incl_sum(nreq);
     const uint32_t soff = send - nreq;
     const bool coop = nreq > 0 && send <= kWaveSegs;
     uint32_t cnt = 0;
     if (coop) {
         uint32_t cx = x0, cy = y0, cz = z0;
-        for (uint32_t s = 0; s < ncell; s += 4) {
+        if (slice) {  // a later slice of a long box
+            const uint32_t nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+            cx += slice % nx;
+            cy += (slice / nx) % ny;
+            cz += slice / (nx * ny);
+        }
+        for (uint32_t s = 0; s < nreq; s += 4) {
             uint32_t k0[4], k1[4];
 #pragma unroll
             for (uint32_t q = 0; q < 4; ++q) {
                 k0[q] = k1[q] = 0;
-                if (s + q < ncell) {
+                if (s + q < nreq) {
                     const uint32_t c = cell_hash(P, cx, cy, cz);
                     k0[q] = P.cell_start[c];
                     k1[q] = P.cell_start[c + 1];
@@ -786,7 +796,7 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
             }
 #pragma unroll
             for (uint32_t q = 0; q < 4; ++q) {
-                if (s + q < ncell) {
+                if (s + q < nreq) {
                     L.seg[soff + s + q] = make_uint2(k0[q] - cnt, cnt + (k1[q] - k0[q]));
                     cnt += k1[q] - k0[q];
                 }
@@ -800,7 +810,7 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
         L.o[lane] = make_float4(ox, oy, oz, __uint_as_float(excluded));
         L.p[lane] = make_float4(px, py, pz, seg_len2(ox, oy, oz, px, py, pz));
         L.meta[lane] = make_uint2(off, soff);
-        L.best[lane] = ~0ull;
+        if (slice == 0) L.best[lane] = ~0ull;  // later slices keep the minimum so far
     }
     uint32_t qlen = 0;  // wave-uniform
     for (uint32_t base = 0; base < total; base += kPass) {
@@ -830,26 +840,13 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
     if (qlen) wave_triangle_pass(P, L, lane, qlen);
     wave_lds_sync();
     key = ~0ull;
-    if (coop) {
+    const bool done = coop && slice + nreq >= ncell;
+    if (coop && !done) slice += nreq;
+    if (done) {
         key = L.best[lane];
         if (key != ~0ull) hit = L.hit[lane];
-    } else if (alone) {
-        Hit best;
-        best.type = 0;
-        best.idx = kNone;
-        best.d = 2.0f;
-        best.px = best.py = best.pz = 0.0f;
-        for (uint32_t cz = z0; cz <= z1; ++cz)
-            for (uint32_t cy = y0; cy <= y1; ++cy)
-                for (uint32_t cx = x0; cx <= x1; ++cx)
-                    cell_test<1>(P, cell_hash(P, cx, cy, cz), excluded, ox, oy, oz, px, py, pz,
-                                 best);
-        if (best.type) {
-            key = hit_key(best.d, best.idx);
-            hit = make_float4(best.px, best.py, best.pz, __uint_as_float(best.type));
-        }
     }
-    return coop || alone;
+    return done;
 }
 
 // Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8),
@@ -902,6 +899,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     float ox = 0.f, oy = 0.f, oz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     float u = 0.f, umax = 1.f, umin = 0.f;
     uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0, nhit = 0;
+    uint32_t slice = 0;  // cells of the segment's box searched so far
     while (__ballot(it < iters) != 0ull) {
         bool complete = false;
         if (it < iters && !pending) {
@@ -923,6 +921,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             umax = 1.0f; umin = 0.0f;
             pending = true;
             refresh = true;
+            slice = 0;
             if (P.cell_clear) {
                 // no triangle lies within `safe` of the path walked since the
                 // last check, so a shorter step cannot hit one
@@ -939,7 +938,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         unsigned long long key;
         float4 hp;
         const bool searched =
-            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, key, hp);
+            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, slice, key, hp);
         if (pending && searched) {
             if (key == ~0ull) {
                 pending = false;
@@ -966,6 +965,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
                 ox = h.px; oy = h.py; oz = h.pz;
                 px = ex; py = ey; pz = ez;
                 excluded = idx;
+                slice = 0;
                 if (ncoll == 4096) {
                     atomicOr(P.flags, 1u);
                     pending = false;

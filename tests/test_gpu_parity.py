@@ -190,6 +190,39 @@ def test_near_grazing_hits_bit_exact():
     np.testing.assert_array_equal(mom, st.moment)
 
 
+@pytest.mark.parametrize("name,scale", [("sim.fibers_1_r004_f09", 6.0), ("sim.triangles_curved", 1.0)])
+def test_injected_physical_mode_fine_grid_bit_exact(ref_inputs, name, scale):
+    # cells of 0.016 against steps of ~0.1 per axis: about a fifth of the cell
+    # boxes hold more than the wave's 256-cell list and are searched in
+    # several slices
+    _say(f"case physical-injected fine grid {name}")
+    m = _mesh(ref_inputs, name, scale)
+    n, iters = 2000, 24
+    dims = (150, 150, 150)
+    lo, cell = (-1.2, -1.2, -1.2), (2.4 / 150,) * 3
+    w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=1, n=n,
+                        std=(0.1,) * 4)
+    pos, comp, fib = _spins(w, m, n, 19, plus_x=True)
+    q = _inject(n, iters, 20)
+    st = O.State(pos, comp=comp, fiber=fib, moment=True)
+    sim.set_state(pos=pos, comp=comp, fiber=fib)
+    sim.moment_reset()
+    tr_o = O.integrate(w, st, iters, rng=O.RNG_INJECT, inject=q, weight=1.0, trace=True)
+    tr_g = sim.integrate_moment_injected(1.0, q, trace=True)
+    g = sim.get_state()
+    mom, _ = sim.get_moment()
+    # the box of a step: cells spanned per axis, from the step's extent
+    ext = np.floor(np.abs(q[..., :3]) * 0.1 / np.array(cell, np.float32)) + 1
+    assert (np.prod(ext, axis=-1) > 256).mean() > 0.1, "too few long boxes to exercise slicing"
+    assert (tr_o[..., 2] > 0).sum() > 200
+    np.testing.assert_array_equal(tr_g, tr_o)
+    np.testing.assert_array_equal(g["pos"], st.pos)
+    np.testing.assert_array_equal(g["mag"], st.mag)
+    np.testing.assert_array_equal(g["comp"], st.comp)
+    np.testing.assert_array_equal(g["fiber"], st.fiber)
+    np.testing.assert_array_equal(mom, st.moment)
+
+
 @pytest.mark.parametrize("name,scale", CASES)
 def test_labels_match_oracle(ref_inputs, name, scale):
     _say(f"case labels {name} scale={scale}")
