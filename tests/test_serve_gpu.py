@@ -140,3 +140,42 @@ def test_background_loop(envs):
         np.testing.assert_array_equal(srv.result(t)[0], want)
     srv.stop()
     srv.close()
+
+
+def test_many_heterogeneous_substrates_in_one_launch():
+    # 24 device-built substrates (20-150 axons, mean radius 0.5-3 um, 32-gon,
+    # cells 0.3-0.6 um), one request on each: all are admitted together and
+    # advanced by the same batched launches, and every result equals the plain
+    # path's bits
+    rng = np.random.default_rng(11)
+    envs, reqs = [], []
+    for k in range(24):
+        cfg = dataclasses.replace(S.SCALE, name=f"het-{k}", n_axons=int(rng.integers(20, 150)),
+                                  mean_radius=float(rng.uniform(0.5, 3.0)), ngon=32,
+                                  fraction=0.5, seed=100 + k)
+        env = S.build(cfg, cell_um=float(rng.uniform(0.3, 0.6)), axons=S.pack(cfg))
+        g = env["grid"]
+        env["sub"] = D.Substrate(env["vertices"], env["triangles"], env["tri_info"], grid="aabb",
+                                 dims=g["dims"], lo=g["lo"], cell=g["cell"], eps=g["eps"],
+                                 device=True)
+        envs.append(env)
+        nl = int(rng.integers(2, 5))
+        grads = (rng.normal(size=(nl, 3)) * 40.0 * 1e-9 * env["space_scale"]).astype(np.float32)
+        reqs.append(dict(env=k, n=int(rng.integers(300, 3000)), seed=2000 + k, dt=cfg.dt,
+                         std_dev=env["std_dev"], t2=env["t2"], permeability=0.01,
+                         lobes=SV.pgse_lobes(int(rng.integers(60, 120)), 300, 20), grads=grads,
+                         boundary=D.BOUNDARY_PERIODIC, extent=(env["xmax"],) * 3,
+                         pc=np.float32(2 * np.pi * GAMMA * cfg.dt)))
+    srv, ids = _server(envs)
+    tickets = [_submit(srv, ids, q) for q in reqs]
+    srv.run_until_idle()
+    stats = []
+    for t, q in zip(tickets, reqs):
+        got, st = srv.result(t)
+        np.testing.assert_array_equal(got, _plain(envs, q))
+        stats.append(st)
+    # every request was admitted before any finished: all shared the launches
+    assert max(s["admit_s"] for s in stats) <= min(s["done_s"] for s in stats)
+    srv.close()
+    for e in envs:
+        e["sub"].close()
