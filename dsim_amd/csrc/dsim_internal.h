@@ -78,6 +78,17 @@ struct LabelParams {
     uint32_t max_tries;
     uint32_t plus_x;  // 1: ray along +x, 0: dominant axis (reference)
     uint32_t *flags;  // bit 1: two odd fibres on one membrane type
+    // First +x ray through the AABB grid (launch_label_row): the row of cells
+    // the ray runs along gives the candidates of the triangles lying inside
+    // the grid box [lo, hi); `outside` lists the others (n_outside on the
+    // device).  Spins left for the tiled pass go to `retry` as (spin, first
+    // try), counted in *n_retry.
+    const uint32_t *cell_start, *cell_tri;
+    uint32_t dims[3];
+    float lo[3], hi[3], cell[3], inv_cell[3];
+    uint32_t *outside, *n_outside;
+    uint2 *retry;
+    uint32_t *n_retry;
 };
 
 enum Rng { kRngInject = 0, kRngMwc = 1, kRngPhilox = 2 };
@@ -107,6 +118,10 @@ hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, 
                                 const float *grads, float pc, double *partial, uint32_t n_blocks,
                                 double *out, hipStream_t s);
 hipError_t launch_label(const LabelParams &p, hipStream_t s);
+// Grid-row labelling of the first +x ray, then the tiled pass over the spins
+// it leaves (ambiguous rays, spins outside the grid).  The buffers of the
+// grid fields of p must be set; *n_outside and *n_retry must be zero.
+hipError_t launch_label_row(const LabelParams &p, hipStream_t s);
 // Batched physical-mode walk over a device-resident job table (periodic or
 // clamped boundary for every job of the launch).
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,

@@ -1457,13 +1457,18 @@ hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, 
 // ------------------------------------------------------------------------ //
 __device__ __forceinline__ uint32_t fiber_hash(uint32_t f) { return f * 0x9E3779B1u + 0x7F4A7C15u; }
 
+// LIST: the spins of P.retry[0, *P.n_retry) from their first try, else all.
+template <bool LIST>
 __global__ __launch_bounds__(256) void k_label(LabelParams P) {
     __shared__ float4 box[2][256];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const bool live = i < P.n;
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t count = LIST ? *P.n_retry : P.n;
+    if (LIST && blockIdx.x * 256 >= count) return;  // block-uniform
+    const bool live = slot < count;
+    const uint32_t i = live && LIST ? P.retry[slot].x : slot;
     float sx = 0.f, sy = 0.f, sz = 0.f;
     if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
-    uint32_t tries = 0;
+    uint32_t tries = live && LIST ? P.retry[slot].y : 0u;
     bool done = !live;
     uint32_t out_comp = 0, out_fiber = kFiberNone;
     while (__syncthreads_or(!done)) {
@@ -1551,7 +1556,136 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
 
 hipError_t launch_label(const LabelParams &p, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_label, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_label<false>, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+// Does a triangle's box start past the grid's +x face?  No grid row lists
+// such a triangle in a cell that a ray from inside the grid box can reach.
+__device__ __forceinline__ bool past_grid(const LabelParams &P, float4 lo) {
+    return lo.x > P.hi[0];
+}
+
+__global__ __launch_bounds__(256) void k_label_outside(LabelParams P) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= P.n_tri) return;
+    if (past_grid(P, P.tri_aabb[2 * t])) P.outside[atomicAdd(P.n_outside, 1u)] = t;
+}
+
+// Cell of coordinate v on axis a whose membership box, lo + c cell to
+// lo + (c+1) cell in float as the grid builds compute it, holds v.
+__device__ __forceinline__ uint32_t label_cell(const LabelParams &P, float v, int a) {
+    uint32_t c = clamp_cell(floorf((v - P.lo[a]) * P.inv_cell[a]), P.dims[a]);
+    while (c > 0 && v < P.lo[a] + (float)c * P.cell[a]) --c;
+    while (c + 1 < P.dims[a] && v > P.lo[a] + (float)(c + 1) * P.cell[a]) ++c;
+    return c;
+}
+
+// Is triangle t listed in cell c (lists are in ascending triangle order)?
+__device__ __forceinline__ bool listed(const LabelParams &P, uint32_t c, uint32_t t) {
+    uint32_t a = P.cell_start[c], b = P.cell_start[c + 1];
+    while (a < b) {
+        const uint32_t m = (a + b) >> 1;
+        const uint32_t v = P.cell_tri[m];
+        if (v == t) return true;
+        if (v < t) a = m + 1; else b = m;
+    }
+    return false;
+}
+
+// The first +x ray of each spin (the tiled kernel's try 0): candidates are the
+// triangles listed in the cells of the ray's grid row from the spin's cell on,
+// each considered once (in the first of those cells that lists it), plus the
+// triangles whose boxes start past the grid's +x face.  The tiled kernel's
+// candidates are the triangles whose boxes hold (x, sy, sz) for some x on the
+// ray; for one that does not start past the grid, x = max(box lo.x, sx) lies
+// in the grid box, so the cell of the row holding x meets the triangle's box
+// and lists it (the grid lists a triangle in every cell whose closed
+// membership box meets its bounding box, and the cells' boxes tile the grid
+// box).  The parity sums do not depend on the order, so the result equals the
+// tiled kernel's first try.  Spins with an ambiguous ray, outside the grid
+// box, or beyond the ray's end are queued for the tiled kernel.
+__global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.n) return;
+    const float sx = P.x[i], sy = P.y[i], sz = P.z[i];
+    const float ex = (float)(1.1 * (double)P.xmax), ey = sy, ez = sz;
+    // the row walk covers rays that start in the grid box and run towards +x
+    if (!(sx >= P.lo[0] && sx <= P.hi[0] && sy >= P.lo[1] && sy <= P.hi[1] && sz >= P.lo[2] &&
+          sz <= P.hi[2] && sx <= ex)) {
+        P.retry[atomicAdd(P.n_retry, 1u)] = make_uint2(i, 0u);
+        return;
+    }
+    const float rminx = fminf(sx, ex), rminy = fminf(sy, ey), rminz = fminf(sz, ez);
+    const float rmaxx = fmaxf(sx, ex), rmaxy = fmaxf(sy, ey), rmaxz = fmaxf(sz, ez);
+    uint32_t xf[3] = {0, 0, 0}, xh[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
+    auto consider = [&](uint32_t t, float4 lo, float4 hi) {
+        if (rminx > hi.x || lo.x > rmaxx || rminy > hi.y || lo.y > rmaxy || rminz > hi.z ||
+            lo.z > rmaxz)
+            return;
+        Hit h;
+        if (P.use_mt)
+            test_mt(P.tri_mt, t, sx, sy, sz, ex, ey, ez, h);
+        else
+            test_ref(P.tri_ref, t, sx, sy, sz, ex, ey, ez, h);
+        if (h.type == 0) return;
+        const uint32_t info = __float_as_uint(lo.w);
+        const uint32_t f = info & 0xFFFFu, m = info >> 16;
+        if (m > 2) return;
+        xf[m] ^= f + 1u;
+        xh[m] ^= fiber_hash(f);
+        if (h.type >= 2) amb[m] += 1;
+    };
+    const uint32_t cx0 = label_cell(P, sx, 0), cy = label_cell(P, sy, 1),
+                   cz = label_cell(P, sz, 2);
+    const uint32_t row = (cz * P.dims[1] + cy) * P.dims[0];
+    for (uint32_t cx = cx0; cx < P.dims[0]; ++cx) {
+        const uint32_t c = row + cx;
+        for (uint32_t k = P.cell_start[c]; k < P.cell_start[c + 1]; ++k) {
+            const uint32_t t = P.cell_tri[k];
+            const float4 lo = P.tri_aabb[2 * t], hi = P.tri_aabb[2 * t + 1];
+            if (past_grid(P, lo)) continue;              // in the outside list
+            if (cx > cx0 && listed(P, c - 1, t)) continue;  // considered one cell back
+            consider(t, lo, hi);
+        }
+    }
+    const uint32_t no = *P.n_outside;
+    for (uint32_t j = 0; j < no; ++j) {
+        const uint32_t t = P.outside[j];
+        consider(t, P.tri_aabb[2 * t], P.tri_aabb[2 * t + 1]);
+    }
+    uint32_t comp = 0, fib = kFiberNone, nedge = 0;
+    bool found = false, multi = false;
+    for (uint32_t m = 0; m < 3 && !found; ++m) {
+        nedge += amb[m];
+        if (xf[m] != 0u) {
+            const uint32_t f = xf[m] - 1u;
+            multi |= fiber_hash(f) != xh[m];
+            found = true;
+            fib = f;
+            comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
+        } else if (xh[m] != 0u) {
+            multi = true;
+        }
+    }
+    if (nedge == 0 && !multi) {
+        P.label[i] = comp | (fib << 8);
+        P.tries[i] = 0;
+    } else if (P.max_tries == 0) {  // no re-cast allowed: keep the first result
+        P.label[i] = comp | (fib << 8);
+        P.tries[i] = 1;
+    } else {
+        P.retry[atomicAdd(P.n_retry, 1u)] = make_uint2(i, 1u);  // re-cast with a jitter
+    }
+}
+
+hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    if (p.n_tri)
+        hipLaunchKernelGGL(k_label_outside, dim3((p.n_tri + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_label_row, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    // the tiled pass over the queued spins; blocks past the queue exit at once
+    hipLaunchKernelGGL(k_label<true>, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -563,9 +563,44 @@ extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
     L.max_tries = max_tries;
     L.plus_x = s->prm.mode == DSIM_MODE_PHYSICAL ? 1u : 0u;
     L.flags = s->flags.p;
-    hipError_t e = dsim::launch_label(L, s->stream);
-    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "label launch: %s", hipGetErrorString(e));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    // +x rays over an AABB grid take the grid-row labeller first; the rest,
+    // and every re-cast, go through the tiled pass over all triangles
+    const dsim_substrate *sub = s->sub;
+    const bool rows = L.plus_x && sub->kind == DSIM_GRID_AABB && sub->n_cells > 0 && s->n > 0;
+    dsim_host::DevBuf<uint32_t> work;  // retry queue, counters, outside list
+    hipError_t e;
+    if (rows) {
+        const size_t q = 2 * (size_t)s->n;  // the queue's uint2 entries sit at the aligned start
+        if (int rc = work.alloc(q + 2 + (size_t)sub->n_tri)) return rc;
+        L.retry = reinterpret_cast<uint2 *>(work.p);
+        L.n_outside = work.p + q;
+        L.n_retry = work.p + q + 1;
+        L.outside = work.p + q + 2;
+        e = hipMemsetAsync(L.n_outside, 0, 8, s->stream);
+        if (e != hipSuccess) {
+            work.release();
+            return fail(DSIM_ERR_HIP, "label: %s", hipGetErrorString(e));
+        }
+        L.cell_start = sub->d_cell_start.p;
+        L.cell_tri = sub->d_cell_tri.p;
+        for (int k = 0; k < 3; ++k) {
+            L.dims[k] = sub->dims[k];
+            L.lo[k] = sub->lo[k];
+            L.cell[k] = sub->cell[k];
+            L.inv_cell[k] = sub->inv_cell[k];
+            L.hi[k] = sub->lo[k] + (float)sub->dims[k] * sub->cell[k];
+        }
+        e = dsim::launch_label_row(L, s->stream);
+    } else {
+        e = dsim::launch_label(L, s->stream);
+    }
+    if (e != hipSuccess) {
+        work.release();
+        return fail(DSIM_ERR_HIP, "label launch: %s", hipGetErrorString(e));
+    }
+    e = hipStreamSynchronize(s->stream);
+    work.release();
+    if (e != hipSuccess) return fail(DSIM_ERR_HIP, "label: %s", hipGetErrorString(e));
     uint32_t fl[4];
     HIP_TRY(hipMemcpy(fl, s->flags.p, 16, hipMemcpyDeviceToHost));
     if (n_ambiguous) {

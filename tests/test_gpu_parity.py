@@ -242,6 +242,40 @@ def test_labels_match_oracle(ref_inputs, name, scale):
         assert len(np.unique(comp)) >= 2
 
 
+ROW_GRIDS = [
+    # fine cells over the whole mesh, no margin
+    dict(dims=(60, 50, 40), lo=(-1.2,) * 3, cell=(0.04, 0.048, 0.06), eps=0.0),
+    # part of the mesh: triangles leaving the grid box take the outside list,
+    # spins outside it the tiled pass over all triangles
+    dict(dims=(17, 11, 7), lo=(-0.3, -0.2, -0.5), cell=(0.03, 0.04, 0.05), eps=2e-5),
+    # a box reaching past 1.1 xmax: spins there cast their ray towards -x
+    dict(dims=(20, 20, 20), lo=(-2.0,) * 3, cell=(0.2,) * 3, eps=1e-5),
+]
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_grid_row_labels_match_oracle(ref_inputs, name, scale):
+    # +x rays over AABB grids go through the grid-row labeller first; the
+    # oracle tests every triangle
+    _say(f"case grid-row labels {name} scale={scale}")
+    m = _mesh(ref_inputs, name, scale)
+    n = 8000
+    for k, g in enumerate(ROW_GRIDS):
+        w, sub, sim = _pair(m, n=n, mode=1, grid="aabb", **g)
+        rng = np.random.default_rng(40 + k)
+        pos = rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+        pos[: n // 2] *= np.float32(0.6)   # half of them near the fibres
+        comp, fib, tries, amb = w.find_spins(pos, m["xmax"], m["ymax"], m["zmax"], plus_x=True)
+        sim.set_state(pos=pos)
+        gamb = sim.label(m["xmax"], m["ymax"], m["zmax"])
+        st = sim.get_state()
+        sim.close()
+        assert gamb == amb, (k, gamb, amb)
+        np.testing.assert_array_equal(st["comp"], comp)
+        np.testing.assert_array_equal(st["fiber"], fib)
+        assert len(np.unique(comp)) >= 2
+
+
 def test_signal_matches_oracle(ref_inputs):
     _say("case signal")
     m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
