@@ -389,13 +389,17 @@ def main():
         f"{prep['identical']}")
     sub, sim = AT.make_sim(env, n, sort_interval=sort_interval)
     _, _, maxc = sub.grid()
-    t0 = time.perf_counter()
     sim.init_uniform(1.0)
-    amb = sim.label(env["xmax"], env["xmax"], env["xmax"])
+    sim.sync()
+    t0 = time.perf_counter()
+    amb = sim.label(env["xmax"], env["xmax"], env["xmax"])   # returns after the device finishes
+    label_s = time.perf_counter() - t0
     st = sim.get_state()
     frac_intra = float(np.mean(st["comp"] == 1))
+    labelling = {"spins": n, "seconds": round(label_s, 4), "ambiguous": int(amb),
+                 "intra_fraction": round(frac_intra, 4)}
     log(f"{len(env['triangles'])} triangles, grid {g['dims']} cell {cell_um} um; labelled {n} "
-        f"spins in {time.perf_counter() - t0:.2f} s: intra {frac_intra:.3f}, ambiguous {amb}")
+        f"spins in {label_s:.3f} s: intra {frac_intra:.3f}, ambiguous {amb}")
 
     prot = env["protocol"]
     grads = (prot[:, 3:6].astype(np.float64) * 1e-9 * env["space_scale"]).astype(np.float32)
@@ -476,6 +480,7 @@ def main():
         "reference_equivalent_value": value * len(prot),
         "autotune": tuned,
         "substrate_prep": prep,
+        "labelling": labelling,
         "roofline": roof,
         "cpu_baseline": cpu,
         "served": served,

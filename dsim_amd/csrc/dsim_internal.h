@@ -77,7 +77,6 @@ struct LabelParams {
     float xmax, ymax, zmax;
     uint32_t max_tries;
     uint32_t plus_x;  // 1: ray along +x, 0: dominant axis (reference)
-    uint32_t *flags;  // bit 1: two odd fibres on one membrane type
     // First +x ray through the AABB grid (launch_label_row): the row of cells
     // the ray runs along gives the candidates of the triangles lying inside
     // the grid box [lo, hi); `outside` lists the others (n_outside on the

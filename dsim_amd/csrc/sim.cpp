@@ -562,7 +562,6 @@ extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
     L.xmax = xmax; L.ymax = ymax; L.zmax = zmax;
     L.max_tries = max_tries;
     L.plus_x = s->prm.mode == DSIM_MODE_PHYSICAL ? 1u : 0u;
-    L.flags = s->flags.p;
     // +x rays over an AABB grid take the grid-row labeller first; the rest,
     // and every re-cast, go through the tiled pass over all triangles
     const dsim_substrate *sub = s->sub;
@@ -601,18 +600,12 @@ extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
     e = hipStreamSynchronize(s->stream);
     work.release();
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "label: %s", hipGetErrorString(e));
-    uint32_t fl[4];
-    HIP_TRY(hipMemcpy(fl, s->flags.p, 16, hipMemcpyDeviceToHost));
     if (n_ambiguous) {
         std::vector<uint32_t> t(s->n);
         if (s->n) HIP_TRY(hipMemcpy(t.data(), s->tries.p, (size_t)s->n * 4, hipMemcpyDeviceToHost));
         uint32_t a = 0;
         for (uint32_t v : t) a += v > max_tries;
         *n_ambiguous = a;
-    }
-    if (fl[0] & 2u) {
-        HIP_TRY(hipMemset(s->flags.p, 0, 16));
-        return fail(DSIM_ERR_OVERFLOW, "two fibres of one membrane type enclose a spin");
     }
     s->moment_ready = false;
     return DSIM_OK;
