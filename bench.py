@@ -353,7 +353,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path is data-parallel over spins and "
               f"would shard by spin id with one FP64 signal all-reduce; "
-              f"--gpus {args.gpus} is not supported in this build.", file=sys.stderr)
+              f"--gpus {args.gpus} is not implemented.", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)

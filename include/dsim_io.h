@@ -1,7 +1,7 @@
 /*
  * dsim I/O: the reference's input and output formats (C ABI).
  *
- * Replaces the driver half of the reference: readInputs and the gradient
+ * Replaces the host-program half of the reference (diffusion.cpp): readInputs and the gradient
  * reader (dSim3/src/diffusion.cpp:124-269), the mesh reader of
  * SpinSystem::initFibers (dSim3/src/spinSystem.cpp:290-610), the PGSE
  * state machine of update() (diffusion.cpp:564-624), computeTensor
