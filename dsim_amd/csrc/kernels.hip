@@ -1455,42 +1455,104 @@ hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, 
 // the candidate order does not matter when at most one fibre per membrane
 // type is crossed an odd number of times (detected otherwise).
 // ------------------------------------------------------------------------ //
+// Longest re-cast queue taken one spin per block (k_label_spin).
+constexpr uint32_t kRetryPerBlockMax = 512;
+
 __device__ __forceinline__ uint32_t fiber_hash(uint32_t f) { return f * 0x9E3779B1u + 0x7F4A7C15u; }
 
-// LIST: the spins of P.retry[0, *P.n_retry) from their first try, else all.
-template <bool LIST>
+// The end point of spin i's ray on try `tries`: to 1.1 * max along +x (the
+// physical mode) or along the spin's dominant axis (findSpinsInFibers), then
+// for re-casts moved by a Philox jitter of up to 0.05 per axis.
+__device__ __forceinline__ void label_ray(const LabelParams &P, uint32_t i, uint32_t tries,
+                                          float sx, float sy, float sz, float &ex, float &ey,
+                                          float &ez) {
+    ex = sx; ey = sy; ez = sz;
+    if (P.plus_x) {
+        ex = (float)(1.1 * (double)P.xmax);
+    } else if ((fabsf(sx) > fabsf(sy)) & (fabsf(sx) > fabsf(sz))) {
+        ex = (float)(((sx >= 0) - (sx < 0)) * 1.1 * (double)P.xmax);
+    } else if (fabsf(sy) > fabsf(sz)) {
+        ey = (float)(((sy >= 0) - (sy < 0)) * 1.1 * (double)P.ymax);
+    } else {
+        ez = (float)(((sz >= 0) - (sz < 0)) * 1.1 * (double)P.zmax);
+    }
+    if (tries > 0) {
+        uint4 o = philox(make_uint4(i, tries, 0u, 0u), 0x6A09E667u, 0xBB67AE85u);
+        ex = (float)((double)ex + (double)u32_unit(o.x) * 0.05);
+        ey = (float)((double)ey + (double)u32_unit(o.y) * 0.05);
+        ez = (float)((double)ez + (double)u32_unit(o.z) * 0.05);
+    }
+}
+
+// One ray's crossing sums: per membrane type the XOR of (fibre + 1) and of the
+// fibre hash over crossed triangles, and the count of edge or vertex hits.
+struct RaySums {
+    uint32_t xf[3], xh[3], amb[3];
+};
+
+// Adds triangle t (box lo/hi, fibre and membrane in lo.w) if its box overlaps
+// the ray's box and the ray crosses it.
+__device__ __forceinline__ void ray_add(const LabelParams &P, uint32_t t, float4 lo, float4 hi,
+                                        float sx, float sy, float sz, float ex, float ey,
+                                        float ez, RaySums &R) {
+    const float rminx = fminf(sx, ex), rminy = fminf(sy, ey), rminz = fminf(sz, ez);
+    const float rmaxx = fmaxf(sx, ex), rmaxy = fmaxf(sy, ey), rmaxz = fmaxf(sz, ez);
+    if (rminx > hi.x || lo.x > rmaxx || rminy > hi.y || lo.y > rmaxy || rminz > hi.z ||
+        lo.z > rmaxz)
+        return;
+    Hit h;
+    if (P.use_mt)
+        test_mt(P.tri_mt, t, sx, sy, sz, ex, ey, ez, h);
+    else
+        test_ref(P.tri_ref, t, sx, sy, sz, ex, ey, ez, h);
+    if (h.type == 0) return;
+    const uint32_t info = __float_as_uint(lo.w);
+    const uint32_t f = info & 0xFFFFu, m = info >> 16;
+    if (m > 2) return;
+    R.xf[m] ^= f + 1u;
+    R.xh[m] ^= fiber_hash(f);
+    if (h.type >= 2) R.amb[m] += 1;
+}
+
+// The label a ray's sums give: the first membrane type (axon, myelin, glia)
+// crossed an odd number of times names the compartment and fibre.  Returns
+// false when the ray is ambiguous (an edge or vertex hit, or two fibres of one
+// membrane type crossed an odd number of times).
+__device__ __forceinline__ bool ray_label(const RaySums &R, uint32_t &comp, uint32_t &fib) {
+    comp = 0;
+    fib = kFiberNone;
+    uint32_t nedge = 0;
+    bool found = false, multi = false;
+    for (uint32_t m = 0; m < 3 && !found; ++m) {
+        nedge += R.amb[m];
+        if (R.xf[m] != 0u) {
+            const uint32_t f = R.xf[m] - 1u;
+            multi |= fiber_hash(f) != R.xh[m];
+            found = true;
+            fib = f;
+            comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
+        } else if (R.xh[m] != 0u) {
+            multi = true;
+        }
+    }
+    return nedge == 0 && !multi;
+}
+
+// Every spin, every triangle: one thread per spin, triangle boxes staged
+// through LDS in tiles of 256.
 __global__ __launch_bounds__(256) void k_label(LabelParams P) {
     __shared__ float4 box[2][256];
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t count = LIST ? *P.n_retry : P.n;
-    if (LIST && blockIdx.x * 256 >= count) return;  // block-uniform
-    const bool live = slot < count;
-    const uint32_t i = live && LIST ? P.retry[slot].x : slot;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < P.n;
     float sx = 0.f, sy = 0.f, sz = 0.f;
     if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
-    uint32_t tries = live && LIST ? P.retry[slot].y : 0u;
+    uint32_t tries = 0;
     bool done = !live;
     uint32_t out_comp = 0, out_fiber = kFiberNone;
     while (__syncthreads_or(!done)) {
-        float ex = sx, ey = sy, ez = sz;
-        if (P.plus_x) {
-            ex = (float)(1.1 * (double)P.xmax);
-        } else if ((fabsf(sx) > fabsf(sy)) & (fabsf(sx) > fabsf(sz))) {
-            ex = (float)(((sx >= 0) - (sx < 0)) * 1.1 * (double)P.xmax);
-        } else if (fabsf(sy) > fabsf(sz)) {
-            ey = (float)(((sy >= 0) - (sy < 0)) * 1.1 * (double)P.ymax);
-        } else {
-            ez = (float)(((sz >= 0) - (sz < 0)) * 1.1 * (double)P.zmax);
-        }
-        if (tries > 0) {
-            uint4 o = philox(make_uint4(i, tries, 0u, 0u), 0x6A09E667u, 0xBB67AE85u);
-            ex = (float)((double)ex + (double)u32_unit(o.x) * 0.05);
-            ey = (float)((double)ey + (double)u32_unit(o.y) * 0.05);
-            ez = (float)((double)ez + (double)u32_unit(o.z) * 0.05);
-        }
-        float rminx = fminf(sx, ex), rminy = fminf(sy, ey), rminz = fminf(sz, ez);
-        float rmaxx = fmaxf(sx, ex), rmaxy = fmaxf(sy, ey), rmaxz = fmaxf(sz, ez);
-        uint32_t xf[3] = {0, 0, 0}, xh[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
+        float ex, ey, ez;
+        label_ray(P, i, tries, sx, sy, sz, ex, ey, ez);
+        RaySums R = {};
         for (uint32_t t0 = 0; t0 < P.n_tri; t0 += 256) {
             __syncthreads();
             uint32_t tl = t0 + threadIdx.x;
@@ -1501,46 +1563,12 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
             __syncthreads();
             if (done) continue;
             uint32_t cnt = min(256u, P.n_tri - t0);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                float4 lo = box[0][k], hi = box[1][k];
-                if (rminx > hi.x || lo.x > rmaxx || rminy > hi.y || lo.y > rmaxy ||
-                    rminz > hi.z || lo.z > rmaxz)
-                    continue;
-                Hit h;
-                if (P.use_mt)
-                    test_mt(P.tri_mt, t0 + k, sx, sy, sz, ex, ey, ez, h);
-                else
-                    test_ref(P.tri_ref, t0 + k, sx, sy, sz, ex, ey, ez, h);
-                if (h.type == 0) continue;
-                uint32_t info = __float_as_uint(lo.w);
-                uint32_t f = info & 0xFFFFu, m = info >> 16;
-                if (m > 2) continue;
-                xf[m] ^= f + 1u;
-                xh[m] ^= fiber_hash(f);
-                if (h.type >= 2) amb[m] += 1;
-            }
+            for (uint32_t k = 0; k < cnt; ++k)
+                ray_add(P, t0 + k, box[0][k], box[1][k], sx, sy, sz, ex, ey, ez, R);
         }
         if (!done) {
-            uint32_t comp = 0, fib = kFiberNone, nedge = 0;
-            bool found = false, multi = false;
-            for (uint32_t m = 0; m < 3 && !found; ++m) {
-                nedge += amb[m];
-                if (xf[m] != 0u) {
-                    uint32_t f = xf[m] - 1u;
-                    multi |= fiber_hash(f) != xh[m];
-                    found = true;
-                    fib = f;
-                    comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
-                } else if (xh[m] != 0u) {
-                    multi = true;
-                }
-            }
-            out_comp = comp;
-            out_fiber = fib;
-            // a ray touching an edge, or finding two odd fibres of one
-            // membrane type (a crossing counted twice or missed at a shared
-            // edge), is re-cast with a jitter
-            if (nedge == 0 && !multi) {
+            const bool clear = ray_label(R, out_comp, out_fiber);
+            if (clear) {
                 done = true;
             } else {
                 tries++;
@@ -1556,7 +1584,7 @@ __global__ __launch_bounds__(256) void k_label(LabelParams P) {
 
 hipError_t launch_label(const LabelParams &p, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_label<false>, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_label, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1593,82 +1621,66 @@ __device__ __forceinline__ bool listed(const LabelParams &P, uint32_t c, uint32_
     return false;
 }
 
-// The first +x ray of each spin (the tiled kernel's try 0): candidates are the
+// Lanes that share one spin's row walk (consecutive lanes of a wave).
+constexpr uint32_t kRowLanes = 8;
+
+// The first +x ray of each spin (k_label's try 0): candidates are the
 // triangles listed in the cells of the ray's grid row from the spin's cell on,
 // each considered once (in the first of those cells that lists it), plus the
-// triangles whose boxes start past the grid's +x face.  The tiled kernel's
-// candidates are the triangles whose boxes hold (x, sy, sz) for some x on the
-// ray; for one that does not start past the grid, x = max(box lo.x, sx) lies
-// in the grid box, so the cell of the row holding x meets the triangle's box
-// and lists it (the grid lists a triangle in every cell whose closed
-// membership box meets its bounding box, and the cells' boxes tile the grid
-// box).  The parity sums do not depend on the order, so the result equals the
-// tiled kernel's first try.  Spins with an ambiguous ray, outside the grid
-// box, or beyond the ray's end are queued for the tiled kernel.
+// triangles whose boxes start past the grid's +x face.  k_label's candidates
+// are the triangles whose boxes hold (x, sy, sz) for some x on the ray; for
+// one that does not start past the grid, x = max(box lo.x, sx) lies in the
+// grid box, so the cell of the row holding x meets the triangle's box and
+// lists it (the grid lists a triangle in every cell whose closed membership
+// box meets its bounding box, and the cells' boxes tile the grid box).  The
+// kRowLanes lanes of a spin take every kRowLanes-th cell and outside triangle
+// and combine their sums; the sums do not depend on the order, so the result
+// equals k_label's first try.  Spins with an ambiguous ray, outside the grid
+// box, or beyond the ray's end are queued for the re-cast kernels.
 __global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= P.n) return;
-    const float sx = P.x[i], sy = P.y[i], sz = P.z[i];
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t i = gid / kRowLanes, g = gid % kRowLanes;
+    const bool live = i < P.n;  // every lane stays for the shuffles below
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
     const float ex = (float)(1.1 * (double)P.xmax), ey = sy, ez = sz;
     // the row walk covers rays that start in the grid box and run towards +x
-    if (!(sx >= P.lo[0] && sx <= P.hi[0] && sy >= P.lo[1] && sy <= P.hi[1] && sz >= P.lo[2] &&
-          sz <= P.hi[2] && sx <= ex)) {
+    const bool row = live && sx >= P.lo[0] && sx <= P.hi[0] && sy >= P.lo[1] &&
+                     sy <= P.hi[1] && sz >= P.lo[2] && sz <= P.hi[2] && sx <= ex;
+    RaySums R = {};
+    if (row) {
+        const uint32_t cx0 = label_cell(P, sx, 0), cy = label_cell(P, sy, 1),
+                       cz = label_cell(P, sz, 2);
+        const uint32_t rowc = (cz * P.dims[1] + cy) * P.dims[0];
+        for (uint32_t cx = cx0 + g; cx < P.dims[0]; cx += kRowLanes) {
+            const uint32_t c = rowc + cx;
+            for (uint32_t k = P.cell_start[c]; k < P.cell_start[c + 1]; ++k) {
+                const uint32_t t = P.cell_tri[k];
+                const float4 lo = P.tri_aabb[2 * t], hi = P.tri_aabb[2 * t + 1];
+                if (past_grid(P, lo)) continue;                 // in the outside list
+                if (cx > cx0 && listed(P, c - 1, t)) continue;  // considered one cell back
+                ray_add(P, t, lo, hi, sx, sy, sz, ex, ey, ez, R);
+            }
+        }
+        const uint32_t no = *P.n_outside;
+        for (uint32_t j = g; j < no; j += kRowLanes) {
+            const uint32_t t = P.outside[j];
+            ray_add(P, t, P.tri_aabb[2 * t], P.tri_aabb[2 * t + 1], sx, sy, sz, ex, ey, ez, R);
+        }
+    }
+    for (uint32_t off = 1; off < kRowLanes; off <<= 1)
+        for (uint32_t m = 0; m < 3; ++m) {
+            R.xf[m] ^= __shfl_xor(R.xf[m], off, 64);
+            R.xh[m] ^= __shfl_xor(R.xh[m], off, 64);
+            R.amb[m] += __shfl_xor(R.amb[m], off, 64);
+        }
+    if (!live || g != 0) return;
+    if (!row) {
         P.retry[atomicAdd(P.n_retry, 1u)] = make_uint2(i, 0u);
         return;
     }
-    const float rminx = fminf(sx, ex), rminy = fminf(sy, ey), rminz = fminf(sz, ez);
-    const float rmaxx = fmaxf(sx, ex), rmaxy = fmaxf(sy, ey), rmaxz = fmaxf(sz, ez);
-    uint32_t xf[3] = {0, 0, 0}, xh[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
-    auto consider = [&](uint32_t t, float4 lo, float4 hi) {
-        if (rminx > hi.x || lo.x > rmaxx || rminy > hi.y || lo.y > rmaxy || rminz > hi.z ||
-            lo.z > rmaxz)
-            return;
-        Hit h;
-        if (P.use_mt)
-            test_mt(P.tri_mt, t, sx, sy, sz, ex, ey, ez, h);
-        else
-            test_ref(P.tri_ref, t, sx, sy, sz, ex, ey, ez, h);
-        if (h.type == 0) return;
-        const uint32_t info = __float_as_uint(lo.w);
-        const uint32_t f = info & 0xFFFFu, m = info >> 16;
-        if (m > 2) return;
-        xf[m] ^= f + 1u;
-        xh[m] ^= fiber_hash(f);
-        if (h.type >= 2) amb[m] += 1;
-    };
-    const uint32_t cx0 = label_cell(P, sx, 0), cy = label_cell(P, sy, 1),
-                   cz = label_cell(P, sz, 2);
-    const uint32_t row = (cz * P.dims[1] + cy) * P.dims[0];
-    for (uint32_t cx = cx0; cx < P.dims[0]; ++cx) {
-        const uint32_t c = row + cx;
-        for (uint32_t k = P.cell_start[c]; k < P.cell_start[c + 1]; ++k) {
-            const uint32_t t = P.cell_tri[k];
-            const float4 lo = P.tri_aabb[2 * t], hi = P.tri_aabb[2 * t + 1];
-            if (past_grid(P, lo)) continue;              // in the outside list
-            if (cx > cx0 && listed(P, c - 1, t)) continue;  // considered one cell back
-            consider(t, lo, hi);
-        }
-    }
-    const uint32_t no = *P.n_outside;
-    for (uint32_t j = 0; j < no; ++j) {
-        const uint32_t t = P.outside[j];
-        consider(t, P.tri_aabb[2 * t], P.tri_aabb[2 * t + 1]);
-    }
-    uint32_t comp = 0, fib = kFiberNone, nedge = 0;
-    bool found = false, multi = false;
-    for (uint32_t m = 0; m < 3 && !found; ++m) {
-        nedge += amb[m];
-        if (xf[m] != 0u) {
-            const uint32_t f = xf[m] - 1u;
-            multi |= fiber_hash(f) != xh[m];
-            found = true;
-            fib = f;
-            comp = (m == 0) ? 1 : (m == 2) ? 3 : 2;
-        } else if (xh[m] != 0u) {
-            multi = true;
-        }
-    }
-    if (nedge == 0 && !multi) {
+    uint32_t comp, fib;
+    if (ray_label(R, comp, fib)) {
         P.label[i] = comp | (fib << 8);
         P.tries[i] = 0;
     } else if (P.max_tries == 0) {  // no re-cast allowed: keep the first result
@@ -1679,13 +1691,115 @@ __global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
     }
 }
 
+// Queued spins of the row labeller, from their recorded try on, when the
+// queue is long: one thread per spin, triangle boxes staged through LDS.
+__global__ __launch_bounds__(256) void k_label_list(LabelParams P) {
+    __shared__ float4 box[2][256];
+    const uint32_t count = *P.n_retry;
+    if (count <= kRetryPerBlockMax || blockIdx.x * 256 >= count) return;  // block-uniform
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    const bool live = slot < count;
+    const uint32_t i = live ? P.retry[slot].x : 0u;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
+    uint32_t tries = live ? P.retry[slot].y : 0u;
+    bool done = !live;
+    uint32_t out_comp = 0, out_fiber = kFiberNone;
+    while (__syncthreads_or(!done)) {
+        float ex, ey, ez;
+        label_ray(P, i, tries, sx, sy, sz, ex, ey, ez);
+        RaySums R = {};
+        for (uint32_t t0 = 0; t0 < P.n_tri; t0 += 256) {
+            __syncthreads();
+            uint32_t tl = t0 + threadIdx.x;
+            if (tl < P.n_tri) {
+                box[0][threadIdx.x] = P.tri_aabb[2 * tl];
+                box[1][threadIdx.x] = P.tri_aabb[2 * tl + 1];
+            }
+            __syncthreads();
+            if (done) continue;
+            uint32_t cnt = min(256u, P.n_tri - t0);
+            for (uint32_t k = 0; k < cnt; ++k)
+                ray_add(P, t0 + k, box[0][k], box[1][k], sx, sy, sz, ex, ey, ez, R);
+        }
+        if (!done) {
+            const bool clear = ray_label(R, out_comp, out_fiber);
+            if (clear) {
+                done = true;
+            } else {
+                tries++;
+                if (tries > P.max_tries) done = true;
+            }
+        }
+    }
+    if (live) {
+        P.label[i] = out_comp | (out_fiber << 8);
+        P.tries[i] = tries;
+    }
+}
+
+// Queued spins when the queue is short: one block per spin, its 256 threads
+// sharing out the triangles, so a spin's re-casts take microseconds.
+__global__ __launch_bounds__(256) void k_label_spin(LabelParams P) {
+    __shared__ uint32_t part[4][9];
+    const uint32_t count = *P.n_retry;
+    if (count > kRetryPerBlockMax) return;  // k_label_list takes a long queue
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {  // block-uniform
+        const uint2 e = P.retry[q];
+        const uint32_t i = e.x;
+        uint32_t tries = e.y;
+        const float sx = P.x[i], sy = P.y[i], sz = P.z[i];
+        uint32_t out_comp = 0, out_fiber = kFiberNone;
+        for (;;) {
+            float ex, ey, ez;
+            label_ray(P, i, tries, sx, sy, sz, ex, ey, ez);
+            RaySums R = {};
+            for (uint32_t t = threadIdx.x; t < P.n_tri; t += 256)
+                ray_add(P, t, P.tri_aabb[2 * t], P.tri_aabb[2 * t + 1], sx, sy, sz, ex, ey, ez,
+                        R);
+            for (uint32_t off = 1; off < 64; off <<= 1)
+                for (uint32_t m = 0; m < 3; ++m) {
+                    R.xf[m] ^= __shfl_xor(R.xf[m], off, 64);
+                    R.xh[m] ^= __shfl_xor(R.xh[m], off, 64);
+                    R.amb[m] += __shfl_xor(R.amb[m], off, 64);
+                }
+            if (lane == 0)
+                for (uint32_t m = 0; m < 3; ++m) {
+                    part[w][m] = R.xf[m];
+                    part[w][3 + m] = R.xh[m];
+                    part[w][6 + m] = R.amb[m];
+                }
+            __syncthreads();
+            RaySums T = {};
+            for (uint32_t v = 0; v < 4; ++v)
+                for (uint32_t m = 0; m < 3; ++m) {
+                    T.xf[m] ^= part[v][m];
+                    T.xh[m] ^= part[v][3 + m];
+                    T.amb[m] += part[v][6 + m];
+                }
+            __syncthreads();  // every thread has read the partials
+            // every thread reaches the same decision from the same sums
+            if (ray_label(T, out_comp, out_fiber)) break;
+            tries++;
+            if (tries > P.max_tries) break;
+        }
+        if (threadIdx.x == 0) {
+            P.label[i] = out_comp | (out_fiber << 8);
+            P.tries[i] = tries;
+        }
+    }
+}
+
 hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
     if (p.n_tri)
         hipLaunchKernelGGL(k_label_outside, dim3((p.n_tri + 255) / 256), dim3(256), 0, s, p);
-    hipLaunchKernelGGL(k_label_row, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
-    // the tiled pass over the queued spins; blocks past the queue exit at once
-    hipLaunchKernelGGL(k_label<true>, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    const uint64_t lanes = (uint64_t)p.n * kRowLanes;
+    hipLaunchKernelGGL(k_label_row, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, p);
+    // the re-casts: the kernel that does not fit the queue's length exits at once
+    hipLaunchKernelGGL(k_label_spin, dim3(kRetryPerBlockMax), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_label_list, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
