@@ -276,6 +276,36 @@ def test_grid_row_labels_match_oracle(ref_inputs, name, scale):
         assert len(np.unique(comp)) >= 2
 
 
+@pytest.mark.parametrize("n", [300, 3000])
+def test_grid_row_label_recasts_match_oracle(ref_inputs, n):
+    # spins level with mesh vertices: their +x rays run through triangle edges,
+    # are re-cast with the jitter, and with a low try limit end ambiguous; a
+    # queue of 300 is re-cast one spin per block, one of 3000 by the tiled
+    # kernel over the queue
+    _say(f"case grid-row label re-casts n={n}")
+    m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
+    V = np.asarray(m["vertices"], np.float32)
+    rng = np.random.default_rng(77 + n)
+    k = rng.integers(0, len(V), n)
+    # y of a vertex: the ray meets the prism's vertical edge through it
+    pos = np.c_[rng.uniform(-1.0, 0.0, n), V[k, 1], rng.uniform(-0.9, 0.9, n)].astype(np.float32)
+    for max_tries in (0, 2, 20):
+        w, sub, sim = _pair(m, n=n, mode=1, grid="aabb", dims=(60, 50, 40), lo=(-1.2,) * 3,
+                            cell=(0.04, 0.048, 0.06), eps=1e-5)
+        comp, fib, tries, amb = w.find_spins(pos, m["xmax"], m["ymax"], m["zmax"],
+                                             max_tries=max_tries, plus_x=True)
+        sim.set_state(pos=pos)
+        gamb = sim.label(m["xmax"], m["ymax"], m["zmax"], max_tries=max_tries)
+        st = sim.get_state()
+        sim.close()
+        assert gamb == amb, (max_tries, gamb, amb)
+        np.testing.assert_array_equal(st["comp"], comp)
+        np.testing.assert_array_equal(st["fiber"], fib)
+        if max_tries == 0:
+            assert amb > n // 10, amb      # most rays touch an edge
+        assert (tries > 0).sum() > n // 10
+
+
 def test_signal_matches_oracle(ref_inputs):
     _say("case signal")
     m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
