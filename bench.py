@@ -181,16 +181,16 @@ def substrate_prep(env):
             "identical": same}
 
 
-def waveform_bench(n_spins, n_wave, sample):
+def waveform_bench(n_spins, n_wave, sample, chunk=256):
     """Waveform-phase path: n_spins spins on the small substrate, n_wave random
     piecewise-constant waveforms (|G| <= 80 mT/m, a new vector every 10 us) over
-    12,000 steps in chunks of 64; the phase GEMM against the BF16 matrix peak;
+    12,000 steps in chunks of `chunk` steps; the phase GEMM against the HBM peak;
     per-waveform error against FP64 on a `sample`-spin run of the same
     substrate and waveforms."""
     from dsim_amd import phase as PH
 
     cfg = S.SMALL
-    steps, chunk = 12000, 64
+    steps = 12000
     t0 = time.perf_counter()
     env = S.build(cfg, cell_um=0.2, axons=S.pack(cfg))
     g = env["grid"]
@@ -257,9 +257,9 @@ def waveform_bench(n_spins, n_wave, sample):
             "gemm_k": 3 * chunk, "substrate": cfg.name, "seconds": wall,
             "spin_steps_per_s": n_spins * steps / wall, "walk_s": walk_ms / 1e3,
             "phase_gemm_s": gemm_ms / 1e3,
-            # at 64 steps per chunk the GEMM is bound by the read-modify-write of
-            # the (phase, compensation) pairs, 8 B per spin and waveform, plus
-            # one read of the chunk's trajectory (4 B per spin and k)
+            # algorithmic bytes: the read-modify-write of the (phase,
+            # compensation) pairs, 8 B per spin and waveform each way, once per
+            # chunk, plus one read of the chunk's trajectory (4 B per spin and k)
             "phase_gemm_roofline": {"bound": "hbm", "achieved": gemm_bytes / (gemm_ms / 1e3) / 1e9,
                                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                                     "frac": gemm_bytes / (gemm_ms / 1e3) / PEAK_HBM,
@@ -347,6 +347,8 @@ def main():
     ap.add_argument("--waveform-spins", type=int, default=4_000_000,
                     help="spins of the waveform-phase measurement (0 skips it)")
     ap.add_argument("--waveforms", type=int, default=1024)
+    ap.add_argument("--chunk-steps", type=int, default=256,
+                    help="steps per phase-GEMM chunk of the waveform run (multiple of 32)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -456,7 +458,7 @@ def main():
                               args.round_steps)
     waves = None
     if args.waveform_spins:
-        waves = waveform_bench(args.waveform_spins, args.waveforms, 8192)
+        waves = waveform_bench(args.waveform_spins, args.waveforms, 8192, args.chunk_steps)
     out = {
         "metric": "spin-steps/s at 1M spins",
         "value": value,

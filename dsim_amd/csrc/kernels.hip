@@ -1691,50 +1691,65 @@ __global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
     }
 }
 
+// Spins of one block of the long-queue re-cast kernel, and lanes per spin.
+constexpr uint32_t kGroupSpins = 16, kGroupLanes = 256 / kGroupSpins;
+
 // Queued spins of the row labeller, from their recorded try on, when the
-// queue is long: one thread per spin, triangle boxes staged through LDS.
-__global__ __launch_bounds__(256) void k_label_list(LabelParams P) {
+// queue is long: kGroupSpins spins per block, kGroupLanes lanes per spin
+// sharing out each LDS tile of triangle boxes, the lanes' sums combined by
+// shuffles after every try.  Blocks take the queue's groups in turn.
+__global__ __launch_bounds__(256) void k_label_group(LabelParams P) {
     __shared__ float4 box[2][256];
     const uint32_t count = *P.n_retry;
-    if (count <= kRetryPerBlockMax || blockIdx.x * 256 >= count) return;  // block-uniform
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-    const bool live = slot < count;
-    const uint32_t i = live ? P.retry[slot].x : 0u;
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
-    uint32_t tries = live ? P.retry[slot].y : 0u;
-    bool done = !live;
-    uint32_t out_comp = 0, out_fiber = kFiberNone;
-    while (__syncthreads_or(!done)) {
-        float ex, ey, ez;
-        label_ray(P, i, tries, sx, sy, sz, ex, ey, ez);
-        RaySums R = {};
-        for (uint32_t t0 = 0; t0 < P.n_tri; t0 += 256) {
-            __syncthreads();
-            uint32_t tl = t0 + threadIdx.x;
-            if (tl < P.n_tri) {
-                box[0][threadIdx.x] = P.tri_aabb[2 * tl];
-                box[1][threadIdx.x] = P.tri_aabb[2 * tl + 1];
+    if (count <= kRetryPerBlockMax) return;  // k_label_spin takes a short queue
+    const uint32_t sp = threadIdx.x / kGroupLanes, g = threadIdx.x % kGroupLanes;
+    for (uint32_t base = blockIdx.x * kGroupSpins; base < count;
+         base += gridDim.x * kGroupSpins) {  // block-uniform
+        const uint32_t slot = base + sp;
+        const bool live = slot < count;
+        const uint2 e = live ? P.retry[slot] : make_uint2(0u, 0u);
+        const uint32_t i = e.x;
+        uint32_t tries = e.y;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
+        bool done = !live;  // uniform over the spin's lanes
+        uint32_t out_comp = 0, out_fiber = kFiberNone;
+        while (__syncthreads_or(!done)) {
+            float ex, ey, ez;
+            label_ray(P, i, tries, sx, sy, sz, ex, ey, ez);
+            RaySums R = {};
+            for (uint32_t t0 = 0; t0 < P.n_tri; t0 += 256) {
+                __syncthreads();
+                const uint32_t tl = t0 + threadIdx.x;
+                if (tl < P.n_tri) {
+                    box[0][threadIdx.x] = P.tri_aabb[2 * tl];
+                    box[1][threadIdx.x] = P.tri_aabb[2 * tl + 1];
+                }
+                __syncthreads();
+                if (done) continue;
+                const uint32_t cnt = min(256u, P.n_tri - t0);
+                for (uint32_t k = g; k < cnt; k += kGroupLanes)
+                    ray_add(P, t0 + k, box[0][k], box[1][k], sx, sy, sz, ex, ey, ez, R);
             }
-            __syncthreads();
-            if (done) continue;
-            uint32_t cnt = min(256u, P.n_tri - t0);
-            for (uint32_t k = 0; k < cnt; ++k)
-                ray_add(P, t0 + k, box[0][k], box[1][k], sx, sy, sz, ex, ey, ez, R);
-        }
-        if (!done) {
-            const bool clear = ray_label(R, out_comp, out_fiber);
-            if (clear) {
-                done = true;
-            } else {
-                tries++;
-                if (tries > P.max_tries) done = true;
+            for (uint32_t off = 1; off < kGroupLanes; off <<= 1)
+                for (uint32_t m = 0; m < 3; ++m) {
+                    R.xf[m] ^= __shfl_xor(R.xf[m], off, 64);
+                    R.xh[m] ^= __shfl_xor(R.xh[m], off, 64);
+                    R.amb[m] += __shfl_xor(R.amb[m], off, 64);
+                }
+            if (!done) {
+                if (ray_label(R, out_comp, out_fiber)) {
+                    done = true;
+                } else {
+                    tries++;
+                    if (tries > P.max_tries) done = true;
+                }
             }
         }
-    }
-    if (live) {
-        P.label[i] = out_comp | (out_fiber << 8);
-        P.tries[i] = tries;
+        if (live && g == 0) {
+            P.label[i] = out_comp | (out_fiber << 8);
+            P.tries[i] = tries;
+        }
     }
 }
 
@@ -1743,7 +1758,7 @@ __global__ __launch_bounds__(256) void k_label_list(LabelParams P) {
 __global__ __launch_bounds__(256) void k_label_spin(LabelParams P) {
     __shared__ uint32_t part[4][9];
     const uint32_t count = *P.n_retry;
-    if (count > kRetryPerBlockMax) return;  // k_label_list takes a long queue
+    if (count > kRetryPerBlockMax) return;  // k_label_group takes a long queue
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {  // block-uniform
         const uint2 e = P.retry[q];
@@ -1799,7 +1814,8 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
     hipLaunchKernelGGL(k_label_row, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, p);
     // the re-casts: the kernel that does not fit the queue's length exits at once
     hipLaunchKernelGGL(k_label_spin, dim3(kRetryPerBlockMax), dim3(256), 0, s, p);
-    hipLaunchKernelGGL(k_label_list, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    const uint32_t groups = (p.n + kGroupSpins - 1) / kGroupSpins;
+    hipLaunchKernelGGL(k_label_group, dim3(groups < 4096u ? groups : 4096u), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -45,7 +45,7 @@ class Waveforms:
     """Phases of n_wave arbitrary gradient waveforms over one walk of `sim`.
     grads: [n_steps, n_wave, 3] gradScaled values."""
 
-    def __init__(self, sim, grads, phase_constant, chunk_steps=64):
+    def __init__(self, sim, grads, phase_constant, chunk_steps=256):
         g = np.ascontiguousarray(grads, np.float32)
         self.n_steps, self.n_wave = g.shape[0], g.shape[1]
         self.n = sim.n

@@ -31,7 +31,9 @@ extern "C" {
 typedef struct dsim_waveforms dsim_waveforms;
 
 /* n_wave: a multiple of 128; n_steps and chunk_steps: multiples of 32 (a
- * chunk of 64 steps gives the GEMM K = 192; the last chunk may be shorter);
+ * chunk of 256 steps gives the GEMM K = 768; the last chunk may be shorter;
+ * longer chunks read and write the phase pairs fewer times: 256 steps take
+ * the 4e6-spin x 1024-waveform product 1.8 times faster than 64 steps);
  * grads: host array
  * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
  * phase_constant = 2*pi*gamma*dt.  Device memory: 8 bytes per (spin,
