@@ -375,7 +375,7 @@ def main():
     cell_um, sort_interval = args.cell_um or 0.25, 128 if args.sort < 0 else args.sort
     if not args.no_autotune and not args.cell_um:
         t0 = time.perf_counter()
-        best, allr = AT.tune(cfg, axons, n, sorts=(0, 128, 256) if args.sort < 0 else (args.sort,),
+        best, allr = AT.tune(cfg, axons, n, sorts=(0, 256, 512, 1024) if args.sort < 0 else (args.sort,),
                              log=log)
         cell_um, sort_interval = best["cell_um"], best["sort_interval"]
         tuned = dict(cell_um=cell_um, sort_interval=sort_interval,
@@ -389,7 +389,8 @@ def main():
     log(f"substrate prep ({prep['triangles']} triangles, {prep['cells']} cells): host build "
         f"{prep['host_s']:.3f} s, device build {prep['device_s']:.3f} s, identical "
         f"{prep['identical']}")
-    sub, sim = AT.make_sim(env, n, sort_interval=sort_interval)
+    sub, sim = AT.make_sim(env, n, sort_interval=sort_interval,
+                           steps_per_launch=AT.launch_steps(sort_interval))
     _, _, maxc = sub.grid()
     sim.init_uniform(1.0)
     sim.sync()
