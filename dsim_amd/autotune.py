@@ -20,8 +20,8 @@ from . import substrate as S
 
 def launch_steps(sort_interval):
     """Steps per step-kernel launch for a reorder interval: one launch per
-    interval (a launch ends when its slowest lane finishes, so longer launches
-    leave fewer lanes idle), 1024 without reordering."""
+    interval (at least 256 steps), 1024 without reordering.  A launch ends when
+    its slowest lane finishes, so longer launches leave lanes idle less often."""
     if sort_interval == 0:
         return 1024
     return max(256, sort_interval)
