@@ -35,7 +35,7 @@ def _mesh(ref_inputs, name, scale):
 
 def _pair(m, *, mode=0, grid="reference", dims=None, lo=None, cell=None, eps=1e-5,
           std=(0.03, 0.03, 0.01, 0.03), t2=(80.0, 70.0, 7.5, 70.0), perm=0.2, boundary=0,
-          rng=D.RNG_PHILOX, n=0, seed=1, sort_interval=0):
+          rng=D.RNG_PHILOX, n=0, seed=1, sort_interval=0, steps_per_launch=0):
     kw = dict(std_dev=std, t2=t2, dt=DT, permeability=perm)
     if grid == "reference":
         w = O.World(m["vertices"], m["trgls"], m["tri_info"], **kw)
@@ -48,7 +48,8 @@ def _pair(m, *, mode=0, grid="reference", dims=None, lo=None, cell=None, eps=1e-
     if mode == 1:
         w.set(test=1, traversal=1, transitions=1, boundary=boundary)
     sim = D.Simulation(sub, n, dt=DT, std_dev=std, t2=t2, permeability=perm, mode=mode,
-                       boundary=boundary, rng=rng, seed=seed, sort_interval=sort_interval)
+                       boundary=boundary, rng=rng, seed=seed, sort_interval=sort_interval,
+                       steps_per_launch=steps_per_launch)
     return w, sub, sim
 
 
@@ -404,16 +405,19 @@ def test_free_diffusion_signal_matches_exp_minus_bD():
 
 
 def test_sorted_philox_run_is_order_independent(ref_inputs):
-    """Spatial reordering changes only where spins live in memory: per-spin
-    results (keyed by spin id) are bit-identical with and without it."""
+    """Spatial reordering and the launch length change only where spins live
+    in memory and how the steps are grouped into launches: per-spin results
+    (keyed by spin id) are bit-identical with and without them."""
     _say("case sort-invariance")
     m = _mesh(ref_inputs, "sim.fibers_1_r004_f09", 6.0)
     n = 20000
     dims, lo, cell = (41, 41, 9), (-1.2, -1.2, -1.2), (2.4 / 41, 2.4 / 41, 2.4 / 9)
     outs = []
-    for sort in (0, 32):
+    # (reorder interval, steps per launch): 100-step launches, 32-step
+    # reorders, 7-step launches, 48-step reorders with long launches
+    for sort, spl in ((0, 0), (32, 0), (0, 7), (48, 1024)):
         w, sub, sim = _pair(m, mode=1, grid="aabb", dims=dims, lo=lo, cell=cell, boundary=1, n=n,
-                            sort_interval=sort, seed=77)
+                            sort_interval=sort, seed=77, steps_per_launch=spl)
         sim.init_uniform(1.0)
         sim.label(m["xmax"], m["ymax"], m["zmax"])
         sim.moment_reset()
@@ -423,9 +427,11 @@ def test_sorted_philox_run_is_order_independent(ref_inputs):
         mom, x0 = sim.get_moment()
         sig = sim.signal_moment(np.array([[0.02, 0.01, 0.0]], np.float32), PC)
         outs.append((st, mom, x0, sig))
-    (a, ma, xa, sa), (b, mb, xb, sb) = outs
-    for k in ("pos", "mag", "comp", "fiber"):
-        np.testing.assert_array_equal(a[k], b[k])
-    np.testing.assert_array_equal(ma, mb)
-    np.testing.assert_array_equal(xa, xb)
-    np.testing.assert_allclose(sa, sb, rtol=1e-12)
+        sim.close()
+    a, ma, xa, sa = outs[0]
+    for b, mb, xb, sb in outs[1:]:
+        for k in ("pos", "mag", "comp", "fiber"):
+            np.testing.assert_array_equal(a[k], b[k])
+        np.testing.assert_array_equal(ma, mb)
+        np.testing.assert_array_equal(xa, xb)
+        np.testing.assert_allclose(sa, sb, rtol=1e-12)
