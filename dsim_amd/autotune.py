@@ -80,7 +80,12 @@ def tune(cfg, axons, n, *, cells_um=(0.15, 0.2, 0.25), sorts=(0, 256, 512, 1024)
                     f"(grid {r['grid']})")
             sim.close()
             sub.close()
+    return pick(results, tie_tol), results
+
+
+def pick(results, tie_tol=0.01):
+    """The candidate to use: among those within tie_tol of the fastest rate,
+    the longest reorder interval, then the largest cell."""
     top = max(r["rate"] for r in results)
     near = [r for r in results if r["rate"] >= (1.0 - tie_tol) * top]
-    best = max(near, key=lambda r: (r["sort_interval"], r["cell_um"]))
-    return best, results
+    return max(near, key=lambda r: (r["sort_interval"], r["cell_um"]))

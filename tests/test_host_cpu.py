@@ -115,3 +115,19 @@ def test_clearance_is_a_distance_lower_bound(ref_inputs, name, scale):
         assert (clr[c] <= d + 1e-12).all(), (c, clr[c], d.min())
         tight += clr[c] > 0.5 * d.min()
     assert tight > 40  # the bound is useful, not just zero
+
+
+def test_autotuner_choice_and_launch_length():
+    # near-ties (within 1 %) go to the longest reorder interval, then the
+    # largest cell; a clearly faster candidate wins outright
+    from dsim_amd import autotune as AT
+    rs = [dict(cell_um=0.15, sort_interval=512, rate=1.650e10),
+          dict(cell_um=0.2, sort_interval=1024, rate=1.640e10),
+          dict(cell_um=0.15, sort_interval=1024, rate=1.645e10),
+          dict(cell_um=0.25, sort_interval=1024, rate=1.500e10)]
+    b = AT.pick(rs)
+    assert (b["cell_um"], b["sort_interval"]) == (0.2, 1024)
+    rs[0]["rate"] = 1.80e10
+    b = AT.pick(rs)
+    assert (b["cell_um"], b["sort_interval"]) == (0.15, 512)
+    assert [AT.launch_steps(s) for s in (0, 128, 256, 512, 1024)] == [1024, 256, 256, 512, 1024]
