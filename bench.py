@@ -153,6 +153,63 @@ def served_bench(n_requests, plain_requests, round_steps=256):
                          f"plain = first {len(sub)} requests one at a time")}
 
 
+def reference_mode_bench(n, steps, cpu_seconds, threads, cpu=True):
+    """The reference-semantics path a drop-in user runs (spinKernelCpu.cpp
+    arithmetic: softsurfer triangle test, first-cell walk over the 50^3 grid,
+    clamp box, MWC seeds; k_step on the device) on the small configuration's
+    mesh: device rate over `steps` steps of `n` spins, and the oracle's rate in
+    the same mode on a sample of the same spins on the host cores."""
+    import oracle as O
+
+    cfg = S.SMALL
+    env = S.build(cfg, cell_um=0.2, axons=S.pack(cfg))
+    V, T, I = env["vertices"], env["triangles"], env["tri_info"]
+    sub = D.Substrate(V, T, I)                       # reference 50^3 grid
+    pc = np.float32(2 * np.pi * GAMMA * cfg.dt)
+    grad = np.array([0.0, 61.7e-9 * env["space_scale"], 0.0], np.float32)
+    sim = D.Simulation(sub, n, dt=cfg.dt, std_dev=env["std_dev"], t2=env["t2"],
+                       permeability=env["permeability"], mode=D.MODE_REFERENCE,
+                       boundary=D.BOUNDARY_CLAMP, rng=D.RNG_MWC, seed=5)
+    sim.init_uniform(1.0)
+    sim.label(env["xmax"], env["xmax"], env["xmax"])
+    sim.integrate(grad, pc, 64)                       # warm-up
+    sim.sync()
+    sim.step_timing_reset()
+    t0 = time.perf_counter()
+    sim.integrate(grad, pc, steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = sim.step_timing()
+    rate = n * steps / wall
+    out = {"spins": n, "steps": steps, "value": rate, "unit": "spin-steps/s",
+           "kernel_ms_per_launch": kernel_ms / max(1, launches), "launches": launches,
+           "substrate": f"{cfg.name} mesh, reference 50^3 grid, clamp box, MWC seeds"}
+    if cpu:
+        k = min(n, 20000)
+        st = sim.get_state(seeds=True)
+        w = O.World(V, T, I, std_dev=env["std_dev"], t2=env["t2"], dt=cfg.dt,
+                    permeability=env["permeability"])
+        s0 = O.State(st["pos"][:k], comp=st["comp"][:k], fiber=st["fiber"][:k],
+                     seed=st["seeds"][:k])
+        t0 = time.perf_counter()
+        O.integrate(w, s0, 20, rng=O.RNG_MWC, grad=grad, phase_constant=pc, n_threads=threads)
+        cal = k * 20 / max(time.perf_counter() - t0, 1e-6)
+        cpu_steps = int(max(20, min(steps, cal * cpu_seconds / k)))
+        t0 = time.perf_counter()
+        O.integrate(w, s0, cpu_steps, rng=O.RNG_MWC, grad=grad, phase_constant=pc,
+                    n_threads=threads)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k * cpu_steps / el, "unit": "spin-steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{k} spins x {cpu_steps} steps, same mode and mesh "
+                                         f"(oracle, {el:.1f} s)"}
+    sim.close()
+    sub.close()
+    log(f"reference mode: {n} spins x {steps} steps at {rate:.3e} spin-steps/s"
+        + (f"; oracle {out['cpu_baseline']['value']:.3e} on {threads} cores" if cpu else ""))
+    return out
+
+
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 matrix, FLOP/s
 PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
 
@@ -347,6 +404,8 @@ def main():
     ap.add_argument("--waveform-spins", type=int, default=4_000_000,
                     help="spins of the waveform-phase measurement (0 skips it)")
     ap.add_argument("--waveforms", type=int, default=1024)
+    ap.add_argument("--reference-spins", type=int, default=1_000_000,
+                    help="spins of the reference-semantics run (0 skips it)")
     ap.add_argument("--chunk-steps", type=int, default=256,
                     help="steps per phase-GEMM chunk of the waveform run (multiple of 32)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -460,6 +519,11 @@ def main():
     waves = None
     if args.waveform_spins:
         waves = waveform_bench(args.waveform_spins, args.waveforms, 8192, args.chunk_steps)
+    refmode = None
+    if args.reference_spins:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        refmode = reference_mode_bench(args.reference_spins, 1000, 5.0, threads,
+                                       cpu=not args.no_cpu_baseline)
     out = {
         "metric": "spin-steps/s at 1M spins",
         "value": value,
@@ -488,6 +552,7 @@ def main():
         "cpu_baseline": cpu,
         "served": served,
         "waveforms": waves,
+        "reference_mode": refmode,
         "host": platform.processor() or platform.machine(),
     }
     print(json.dumps(out), flush=True)
