@@ -851,22 +851,23 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
 
 // Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8),
 // and each XCD has its own L2.  The spins are renumbered so that each XCD
-// takes runs of kXcdRun consecutive spin blocks: once the spins are sorted
+// takes runs of `run` consecutive spin blocks: once the spins are sorted
 // (rate band, then Morton order) the blocks an XCD holds at a time cover a
-// compact region, whose triangles stay in its L2.  Blocks past the last whole
-// round of runs keep their number.
+// compact stretch of that order.  Blocks past the last whole round of runs
+// keep their number.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kXcdRun = 8;
-__device__ __forceinline__ uint32_t xcd_block() {
+constexpr uint32_t kWalkXcdRun = 128;  // k_walk: one-wave blocks (8192 spins a run)
+constexpr uint32_t kBatchXcdRun = 8;   // k_walk_batch: four-wave blocks (2048 spins a run)
+__device__ __forceinline__ uint32_t xcd_block(uint32_t run) {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t full = nb / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+    const uint32_t full = nb / (kXcds * run) * (kXcds * run);
     if (b >= full) return b;
     const uint32_t x = b % kXcds, k = b / kXcds;  // XCD and its k-th workgroup
-    return ((k / kXcdRun) * kXcds + x) * kXcdRun + k % kXcdRun;
+    return ((k / run) * kXcds + x) * run + k % run;
 }
 
 // cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode: `iters` steps
-// for the 256 spins of block `block` of the simulation described by P.
+// for the blockDim.x spins of block `block` of the simulation described by P.
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
                                           WaveScratch &L) {
@@ -1035,10 +1036,13 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
 }
 
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
+// One wave per workgroup: every wave runs until its own slowest lane is done,
+// and a finished wave frees its slot for the next block at once (with four
+// waves per workgroup the slot waited for the slowest of the four).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
     StepParams P, uint32_t iters) {
-    __shared__ WaveScratch scratch[4];
-    walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(), iters, scratch[threadIdx.x >> 6]);
+    __shared__ WaveScratch scratch[1];
+    walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
 }
 
 // Batched walk for the server: the blocks of one launch are shared out over
@@ -1051,7 +1055,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 template <bool PERIODIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_batch(const WalkTable *__restrict__ T) {
     __shared__ WaveScratch scratch[4];
-    const uint32_t b = xcd_block();
+    const uint32_t b = xcd_block(kBatchXcdRun);
     if (b >= T->total_blocks) return;
     uint32_t j = 0;
     for (uint32_t st = kMaxJobs / 2; st; st >>= 1)
@@ -1073,11 +1077,12 @@ hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool 
 
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
-    dim3 grid((p.n + 255) / 256), block(256);
     if constexpr (MODE == 0) {
-        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
+        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 255) / 256),
+                           dim3(256), 0, s, p, iters);
     } else {
-        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), grid, block, 0, s, p, iters);
+        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 63) / 64),
+                           dim3(64), 0, s, p, iters);
     }
 }
 
