@@ -96,6 +96,10 @@ enum Rng { kRngInject = 0, kRngMwc = 1, kRngPhilox = 2 };
 // job j advances the simulation of job[j].p by job[j].iters steps on blocks
 // [block_start[j], block_start[j+1]).
 constexpr uint32_t kMaxJobs = 256;
+// Spins per workgroup of the physical-mode step kernels (k_walk and
+// k_walk_batch): one wave.
+constexpr uint32_t kWalkBlockSpins = 64;
+
 struct WalkJob {
     StepParams p;
     uint32_t iters;

@@ -856,8 +856,8 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
 // compact stretch of that order.  Blocks past the last whole round of runs
 // keep their number.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kWalkXcdRun = 128;  // k_walk: one-wave blocks (8192 spins a run)
-constexpr uint32_t kBatchXcdRun = 8;   // k_walk_batch: four-wave blocks (2048 spins a run)
+constexpr uint32_t kWalkXcdRun = 128;  // k_walk: 8192 spins a run
+constexpr uint32_t kBatchXcdRun = 32;  // k_walk_batch: 2048 spins a run (jobs are small)
 __device__ __forceinline__ uint32_t xcd_block(uint32_t run) {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t full = nb / (kXcds * run) * (kXcds * run);
@@ -1039,7 +1039,7 @@ template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 // One wave per workgroup: every wave runs until its own slowest lane is done,
 // and a finished wave frees its slot for the next block at once (with four
 // waves per workgroup the slot waited for the slowest of the four).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
+__global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
     StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[1];
     walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
@@ -1053,25 +1053,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // table's total exit at once, so one launch shape (and one captured graph)
 // serves any mix of jobs.
 template <bool PERIODIC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_batch(const WalkTable *__restrict__ T) {
-    __shared__ WaveScratch scratch[4];
+__global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_batch(const WalkTable *__restrict__ T) {
+    __shared__ WaveScratch scratch[1];
     const uint32_t b = xcd_block(kBatchXcdRun);
     if (b >= T->total_blocks) return;
     uint32_t j = 0;
     for (uint32_t st = kMaxJobs / 2; st; st >>= 1)
         if (j + st < T->n_jobs && T->block_start[j + st] <= b) j += st;
     const WalkJob &J = T->job[j];
-    walk_body<PERIODIC, kRngPhilox, false, true>(J.p, b - T->block_start[j], J.iters,
-                                                 scratch[threadIdx.x >> 6]);
+    walk_body<PERIODIC, kRngPhilox, false, true>(J.p, b - T->block_start[j], J.iters, scratch[0]);
 }
 
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
                              hipStream_t s) {
     if (grid_blocks == 0) return hipSuccess;
     if (periodic)
-        hipLaunchKernelGGL(k_walk_batch<true>, dim3(grid_blocks), dim3(256), 0, s, table);
+        hipLaunchKernelGGL(k_walk_batch<true>, dim3(grid_blocks), dim3(kWalkBlockSpins), 0, s, table);
     else
-        hipLaunchKernelGGL(k_walk_batch<false>, dim3(grid_blocks), dim3(256), 0, s, table);
+        hipLaunchKernelGGL(k_walk_batch<false>, dim3(grid_blocks), dim3(kWalkBlockSpins), 0, s,
+                           table);
     return hipGetLastError();
 }
 
@@ -1081,8 +1081,9 @@ static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
         hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 255) / 256),
                            dim3(256), 0, s, p, iters);
     } else {
-        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 63) / 64),
-                           dim3(64), 0, s, p, iters);
+        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>),
+                           dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
+                           dim3(kWalkBlockSpins), 0, s, p, iters);
     }
 }
 

@@ -151,8 +151,12 @@ void release(dsim_server *S, Req *r) {
 }
 
 // Workgroups of the step kernel resident at once on the device (256 CUs, four
-// 256-spin workgroups each at its register and LDS use).
-constexpr uint32_t kResidentBlocks = 1024;
+// one-wave workgroups per SIMD at its register use).
+constexpr uint32_t kResidentBlocks = 4096;
+
+uint32_t walk_blocks(uint32_t n_spins) {
+    return (n_spins + dsim::kWalkBlockSpins - 1) / dsim::kWalkBlockSpins;
+}
 
 int ensure_batch(dsim_server *S, int kind) {
     Batch &B = S->batch[kind];
@@ -194,7 +198,7 @@ int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
             if (!r->ready || r->read_out || r->r.boundary != (uint32_t)kind || r->lobe_left == 0)
                 continue;
             jobs.push_back(r);
-            total += (r->r.n_spins + 255) / 256;
+            total += walk_blocks(r->r.n_spins);
         }
         std::stable_sort(jobs.begin(), jobs.end(),
                          [](const Req *a, const Req *b) { return a->r.n_spins < b->r.n_spins; });
@@ -206,14 +210,14 @@ int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
                 if (int rc = ensure_batch(S, kind)) return rc;
                 T = S->batch[kind].host;
             }
-            const uint32_t nb = (r->r.n_spins + 255) / 256;
+            const uint32_t nb = walk_blocks(r->r.n_spins);
             const uint32_t want = nb * 8 <= total ? S->round_steps * stretch : S->round_steps;
             r->chunk = std::min(want, r->lobe_left);
             dsim::WalkJob &J = T->job[nj];
             if (int rc = dsim_host::sim_batch_params(r->sim, r->lobe_w[r->lobe], &J.p)) return rc;
             J.iters = r->chunk;
             T->block_start[nj] = blocks;
-            blocks += (r->r.n_spins + 255) / 256;
+            blocks += nb;
             nj++;
         }
         if (!nj) continue;
@@ -326,7 +330,7 @@ extern "C" int dsim_server_create(uint64_t hbm_budget, uint32_t round_steps,
     S->budget = hbm_budget;
     S->round_steps = round_steps ? round_steps : 256;
     S->max_spins = max_active_spins ? max_active_spins : (64u << 20);
-    S->grid_blocks = (S->max_spins + 255) / 256 + dsim::kMaxJobs;
+    S->grid_blocks = walk_blocks(S->max_spins) + dsim::kMaxJobs;
     hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete S;
