@@ -480,7 +480,7 @@ __device__ __forceinline__ float clearance_at(const StepParams &P, float x, floa
 // `iters` steps per launch with the spin's state held in registers.
 // ------------------------------------------------------------------------ //
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
-__global__ __launch_bounds__(256) void k_step(StepParams P, uint32_t iters) {
+__global__ __launch_bounds__(64) void k_step(StepParams P, uint32_t iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     // the spin's identity keys its random stream, injected normals and trace
@@ -1078,8 +1078,8 @@ hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool 
 template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     if constexpr (MODE == 0) {
-        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 255) / 256),
-                           dim3(256), 0, s, p, iters);
+        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 63) / 64),
+                           dim3(64), 0, s, p, iters);
     } else {
         hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>),
                            dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
