@@ -591,7 +591,9 @@ __global__ __launch_bounds__(64) void k_step(StepParams P, uint32_t iters) {
 // ------------------------------------------------------------------------ //
 constexpr uint32_t kWaveSegs = 256;   // cell ranges listed per wave and search round
 constexpr uint32_t kPass = 128;       // pairs per pass: two per lane, loads in flight together
-constexpr uint32_t kWaveQueue = 192;  // plane crossings awaiting the triangle test
+// plane crossings awaiting the triangle test; the scratch of 16 waves (four
+// per SIMD) then takes 155.6 KB of a CU's 160 KB LDS
+constexpr uint32_t kWaveQueue = 384;
 
 struct WaveScratch {
     uint2 seg[kWaveSegs];  // (first entry - lane-local pair number, lane-local end)
