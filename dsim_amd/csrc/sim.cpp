@@ -29,7 +29,7 @@ enum Field {
     F_COUNT = F_HITS + 1
 };
 
-const uint32_t kDefaultStepsPerLaunch = 256;
+const uint32_t kDefaultStepsPerLaunch = 1024;
 
 }  // namespace
 

@@ -97,7 +97,8 @@ typedef struct {
     uint32_t sort_interval;   /* physical mode with Philox: reorder spins in device memory
                                * every this many steps (0 = never), by recent membrane-hit
                                * rate, then cell.  Results do not depend on it. */
-    uint32_t steps_per_launch; /* steps fused per kernel launch (0 = 256) */
+    uint32_t steps_per_launch; /* steps fused per kernel launch (0 = 1024; with reordering at
+                                  most sort_interval) */
 } dsim_params;
 
 typedef struct dsim_substrate dsim_substrate;
