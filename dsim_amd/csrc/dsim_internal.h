@@ -41,6 +41,8 @@ struct StepParams {
     float perm;
     uint32_t reflection_type;
     float clamp_hi, clamp_lo;
+    // periodic voxel [wrap_lo, wrap_hi) per axis, period wrap_len = hi - lo
+    float wrap_lo[3], wrap_hi[3], wrap_len[3];
     // spin state (device pointers, SoA)
     uint32_t n;
     float *x, *y, *z, *mag, *phase;
@@ -51,6 +53,8 @@ struct StepParams {
     float *mom;       // [3][n] moment
     float *momc;      // [3][n] Kahan compensation
     uint32_t *hits;   // membrane hits per spin since the last reorder, or null
+    uint32_t *wrap;   // periodic voxel crossings per spin (three signed 10-bit counts), used
+                      // for the phase of the unwrapped position when moments are off
     uint32_t hit_window;  // steps those counts cover (0: no history)
     float *traj;          // [3*iters][traj_stride] displacement after each step of the
                           // launch, by spin id (moment mode), or null
@@ -76,12 +80,12 @@ struct LabelParams {
     uint32_t *tries;
     float xmax, ymax, zmax;
     uint32_t max_tries;
-    uint32_t plus_x;  // 1: ray along +x, 0: dominant axis (reference)
-    // First +x ray through the AABB grid (launch_label_row): the row of cells
-    // the ray runs along gives the candidates of the triangles lying inside
-    // the grid box [lo, hi); `outside` lists the others (n_outside on the
-    // device).  Spins left for the tiled pass go to `retry` as (spin, first
-    // try), counted in *n_retry.
+    uint32_t ray_axis;  // 1, 2, 3: ray along +x, +y, +z; 0: dominant axis (reference)
+    // First ray through an AABB or exact-membership grid (launch_label_row):
+    // the row of cells the ray runs along gives the candidates of the
+    // triangles lying inside the grid box [lo, hi); `outside` lists the others
+    // (n_outside on the device).  Spins left for the tiled pass go to `retry`
+    // as (spin, first try), counted in *n_retry.
     const uint32_t *cell_start, *cell_tri;
     uint32_t dims[3];
     float lo[3], hi[3], cell[3], inv_cell[3];
@@ -130,8 +134,8 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s);
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
                              hipStream_t s);
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
-                               uint32_t *label, uint32_t *id, float start_box, uint32_t key0,
-                               uint32_t key1, hipStream_t s);
+                               uint32_t *label, uint32_t *id, uint32_t *wrap, const float lo[3],
+                               const float hi[3], uint32_t key0, uint32_t key1, hipStream_t s);
 hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
                                float *x0, float *disp, float *mom, float *momc, float *mag,
                                hipStream_t s);

@@ -29,6 +29,7 @@
 
 #include "../../include/dsim.h"
 #include "host_common.h"
+#include "tribox.h"
 
 using dsim_host::DevBuf;
 using dsim_host::fail;
@@ -234,6 +235,21 @@ __device__ void cells_of(const Geom &g, const float *V, const uint32_t *T, uint3
                 const float bx0 = g.lo[0] + (float)x * g.cell[0],
                             bx1 = g.lo[0] + (float)(x + 1) * g.cell[0];
                 if (bx0 > mx[0] || mn[0] > bx1) continue;
+                if (g.kind == DSIM_GRID_TRIBOX) {
+                    // exact membership: host.cpp tri_cell_separated
+                    double a[3], b[3], c[3], b0[3], b1[3];
+                    const int64_t idx[3] = {x, y, z};
+                    for (int k = 0; k < 3; ++k) {
+                        a[k] = V[T[t * 3] * 3 + k];
+                        b[k] = V[T[t * 3 + 1] * 3 + k];
+                        c[k] = V[T[t * 3 + 2] * 3 + k];
+                        b0[k] = (double)g.lo[k] + (double)idx[k] * (double)g.cell[k];
+                        b1[k] = (double)g.lo[k] + (double)(idx[k] + 1) * (double)g.cell[k];
+                    }
+                    dsim_host::TriBoxAxes X;
+                    dsim_host::tribox_setup(a, b, c, b0, b1, X);
+                    if (dsim_host::tribox_separated(X, (double)g.eps)) continue;
+                }
                 fn((uint32_t)((z * g.dims[1] + y) * g.dims[0] + x));
             }
         }
@@ -324,7 +340,8 @@ __device__ float float_sqrt_floor(double x) {
 // the triangles of the rings around the cell.
 __global__ __launch_bounds__(kBlock) void k_clearance(Geom g, const uint32_t *ring,
                                                       const uint32_t *start, const uint32_t *tri,
-                                                      const float4 *box, float *out) {
+                                                      const float4 *box, const float *V,
+                                                      const uint32_t *T, float *out) {
     const int64_t D[3] = {g.dims[0], g.dims[1], g.dims[2]};
     const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= (uint64_t)D[0] * D[1] * D[2]) return;
@@ -362,14 +379,28 @@ __global__ __launch_bounds__(kBlock) void k_clearance(Geom g, const uint32_t *ri
                     if (imax(imax(ax, ay), az) != r) continue;  // the ring at radius r only
                     const uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
                     for (uint32_t k = start[cc]; k < start[cc + 1]; ++k) {
-                        const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
-                        const double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z};
                         double d2 = 0;
-                        for (int q = 0; q < 3; ++q) {
-                            double gap = 0.0;
-                            if (lo3[q] - b1[q] > gap) gap = lo3[q] - b1[q];
-                            if (b0[q] - hi3[q] > gap) gap = b0[q] - hi3[q];
-                            d2 += gap * gap;
+                        if (g.kind == DSIM_GRID_TRIBOX) {
+                            // host.cpp: the largest separating-axis gap
+                            const uint32_t t = tri[k];
+                            double a[3], b[3], c[3];
+                            for (int q = 0; q < 3; ++q) {
+                                a[q] = V[T[t * 3] * 3 + q];
+                                b[q] = V[T[t * 3 + 1] * 3 + q];
+                                c[q] = V[T[t * 3 + 2] * 3 + q];
+                            }
+                            dsim_host::TriBoxAxes X;
+                            dsim_host::tribox_setup(a, b, c, b0, b1, X);
+                            d2 = dsim_host::tribox_gap2(X);
+                        } else {
+                            const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
+                            const double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z};
+                            for (int q = 0; q < 3; ++q) {
+                                double gap = 0.0;
+                                if (lo3[q] - b1[q] > gap) gap = lo3[q] - b1[q];
+                                if (b0[q] - hi3[q] > gap) gap = b0[q] - hi3[q];
+                                d2 += gap * gap;
+                            }
                         }
                         if (d2 < best2) best2 = d2;
                     }
@@ -405,7 +436,7 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
         for (int k = 0; k < 3; ++k) {
             S->dims[k] = n; S->lo[k] = -1.0f; S->cell[k] = L; S->inv_cell[k] = 1.0f / L;
         }
-    } else if (grid->kind == DSIM_GRID_AABB) {
+    } else if (grid->kind == DSIM_GRID_AABB || grid->kind == DSIM_GRID_TRIBOX) {
         for (int k = 0; k < 3; ++k) {
             if (grid->dims[k] == 0 || !(grid->cell[k] > 0.0f)) {
                 delete S;
@@ -433,6 +464,7 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
         if (mesh->tri_info[t * 3 + 1] == 1) myel[mesh->tri_info[t * 3]] = 1;
     S->on_device = true;
     S->device_built = true;
+    S->label_axis = dsim_host::label_axis_of(mesh);
 
     int rc = DSIM_OK;
     const hipStream_t st = nullptr;
@@ -541,9 +573,13 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
     GRID_TRY(hipMemcpyAsync(&S->max_per_cell, maxc.p, 4, hipMemcpyDeviceToHost, st));
     S->n_entries = n_pairs;
 
-    // cell-major records
-    if ((rc = S->cell_plane.alloc(np1)) || (rc = S->cell_mt.alloc(np1 * 3))) return bail(rc);
-    if (n_pairs) {
+    // cell-major records (not for exact-membership grids: their long
+    // triangles are listed in many cells, so entries keep only the index)
+    const bool cell_major = grid->kind != DSIM_GRID_TRIBOX;
+    if (cell_major && ((rc = S->cell_plane.alloc(np1)) || (rc = S->cell_mt.alloc(np1 * 3))))
+        return bail(rc);
+    if (!cell_major) {
+    } else if (n_pairs) {
         hipLaunchKernelGGL(k_cell_records, dim3(blocks_for(n_pairs)), dim3(kBlock), 0, st,
                            n_pairs, S->d_cell_tri.p, S->tri_mt.p, S->cell_plane.p, S->cell_mt.p);
         GRID_TRY(hipGetLastError());
@@ -552,8 +588,8 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
         GRID_TRY(hipMemsetAsync(S->cell_mt.p, 0, 48, st));
     }
 
-    // clearance (AABB grids)
-    if (grid->kind == DSIM_GRID_AABB) {
+    // clearance (AABB and exact-membership grids)
+    if (grid->kind != DSIM_GRID_REFERENCE) {
         if ((rc = ring.alloc(S->n_cells)) || (rc = S->d_clearance.alloc(S->n_cells)))
             return bail(rc);
         const uint32_t nb = blocks_for(S->n_cells);
@@ -575,7 +611,8 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
             if (!changed) break;
         }
         hipLaunchKernelGGL(k_clearance, dim3(nb), dim3(kBlock), 0, st, g, ring.p,
-                           S->d_cell_start.p, S->d_cell_tri.p, S->tri_aabb.p, S->d_clearance.p);
+                           S->d_cell_start.p, S->d_cell_tri.p, S->tri_aabb.p, dV.p, dT.p,
+                           S->d_clearance.p);
         GRID_TRY(hipGetLastError());
     }
     GRID_TRY(hipStreamSynchronize(st));

@@ -18,6 +18,7 @@
 #include "../../include/dsim.h"
 #include "dsim_internal.h"
 #include "host_common.h"
+#include "tribox.h"
 
 namespace dsim_host {
 
@@ -186,6 +187,31 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
     for (const Pair &p : pairs) tri[fill[p.cell]++] = p.tri;
 }
 
+// Vertices of triangle t in double.
+inline void tri_vertices(const float *V, const uint32_t *T, uint32_t t, double a[3], double b[3],
+                         double c[3]) {
+    for (int k = 0; k < 3; ++k) {
+        a[k] = V[T[t * 3] * 3 + k];
+        b[k] = V[T[t * 3 + 1] * 3 + k];
+        c[k] = V[T[t * 3 + 2] * 3 + k];
+    }
+}
+
+// Exact-membership test of triangle t against cell idx (its box in double
+// from the grid origin and cell edges).
+inline bool tri_cell_separated(const float *V, const uint32_t *T, uint32_t t, const float lo[3],
+                               const float ce[3], const int64_t idx[3], float eps) {
+    double a[3], b[3], c[3], b0[3], b1[3];
+    tri_vertices(V, T, t, a, b, c);
+    for (int k = 0; k < 3; ++k) {
+        b0[k] = (double)lo[k] + (double)idx[k] * (double)ce[k];
+        b1[k] = (double)lo[k] + (double)(idx[k] + 1) * (double)ce[k];
+    }
+    dsim_host::TriBoxAxes X;
+    dsim_host::tribox_setup(a, b, c, b0, b1, X);
+    return dsim_host::tribox_separated(X, (double)eps);
+}
+
 // Per-cell clearance: a lower bound of the distance from any point of the cell
 // to any triangle registered in the grid.  A breadth-first pass over the
 // 26-neighbourhood gives each cell its Chebyshev ring distance D to the
@@ -196,9 +222,13 @@ void pairs_to_csr(const std::vector<Pair> &pairs, uint32_t n_cells, std::vector<
 // (growing Chebyshev radius, stopping once a ring is farther than the best
 // found).  Rounded down to float; 1e30 when the grid holds no triangle.  The
 // device build (grid.hip) computes the same values.
+// dist2(t, b0, b1) is a lower bound of the squared distance between triangle
+// t and the box [b0, b1]: the distance between their bounding boxes (AABB
+// grids) or the largest separating-axis gap (exact-membership grids).
+template <class Dist2>
 void compute_clearance(const uint32_t dims[3], const float lo[3], const float cell[3],
                        const std::vector<uint32_t> &start, const std::vector<uint32_t> &tri,
-                       const std::vector<float4> &box, std::vector<float> &out) {
+                       Dist2 dist2, std::vector<float> &out) {
     constexpr uint32_t kExactRings = 3;
     constexpr uint32_t kFar = 0xFFFFFFFFu;
     const uint64_t nc = (uint64_t)dims[0] * dims[1] * dims[2];
@@ -255,15 +285,8 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
                                     if (std::max({std::llabs(x - ix), std::llabs(y - iy), std::llabs(z - iz)}) != r)
                                         continue;  // only the ring at Chebyshev radius r
                                     uint64_t cc = ((uint64_t)z * D[1] + y) * D[0] + x;
-                                    for (uint32_t k = start[cc]; k < start[cc + 1]; ++k) {
-                                        const float4 a = box[(size_t)tri[k] * 2], b = box[(size_t)tri[k] * 2 + 1];
-                                        double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z}, d2 = 0;
-                                        for (int q = 0; q < 3; ++q) {
-                                            double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
-                                            d2 += g * g;
-                                        }
-                                        best2 = std::min(best2, d2);
-                                    }
+                                    for (uint32_t k = start[cc]; k < start[cc + 1]; ++k)
+                                        best2 = std::min(best2, dist2(tri[k], b0, b1));
                                 }
                     }
                     out[c] = dsim_host::float_sqrt_floor(best2);
@@ -299,6 +322,41 @@ int dsim_host::check_mesh(const dsim_mesh *mesh) {
         if (mesh->tri_info[t * 3 + 2] >= nV) return fail(DSIM_ERR_ARG, "triangle %u P1 bad", t);
     }
     return DSIM_OK;
+}
+
+// The axis of the physical mode's labelling rays.  A ray that runs along an
+// open tube never crosses its wall, so the ray must not run along the mesh's
+// long walls: per axis, sum the extent along that axis of the triangles
+// lying parallel to it (normal within 1e-3 of perpendicular).  +x is kept
+// while that sum is at most 5% of the triangles' total extent (z-aligned
+// prisms, capped or not); otherwise the axis of the smaller sum of y and z.
+uint32_t dsim_host::label_axis_of(const dsim_mesh *mesh) {
+    const float *V = mesh->vertices;
+    const uint32_t *T = mesh->triangles;
+    double score[3] = {0, 0, 0}, total = 0;
+    for (uint32_t t = 0; t < mesh->n_triangles; ++t) {
+        double a[3], b[3], c[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = V[T[t * 3] * 3 + k];
+            b[k] = V[T[t * 3 + 1] * 3 + k];
+            c[k] = V[T[t * 3 + 2] * 3 + k];
+        }
+        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                             u[0] * v[1] - u[1] * v[0]};
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        double ext[3], emax = 0;
+        for (int k = 0; k < 3; ++k) {
+            ext[k] = std::max({a[k], b[k], c[k]}) - std::min({a[k], b[k], c[k]});
+            emax = std::max(emax, ext[k]);
+        }
+        total += emax;
+        for (int k = 0; k < 3; ++k)
+            if (std::fabs(n[k]) < 1e-3 * nn) score[k] += ext[k];
+    }
+    if (score[0] <= 0.05 * total) return 0;
+    return score[2] < score[1] ? 2 : 1;
 }
 
 extern "C" const char *dsim_last_error(void) { return dsim_host::g_err.c_str(); }
@@ -408,7 +466,8 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
                             o.push_back(Pair{(zc * n + yc) * n + xc, t});
                     }
         });
-    } else if (grid->kind == DSIM_GRID_AABB) {
+    } else if (grid->kind == DSIM_GRID_AABB || grid->kind == DSIM_GRID_TRIBOX) {
+        const bool exact = grid->kind == DSIM_GRID_TRIBOX;
         for (int k = 0; k < 3; ++k) {
             if (grid->dims[k] == 0 || !(grid->cell[k] > 0.0f)) {
                 delete S;
@@ -447,6 +506,10 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
                     for (int64_t x = i0[0]; x <= i1[0]; ++x) {
                         float bx0 = lo[0] + (float)x * ce[0], bx1 = lo[0] + (float)(x + 1) * ce[0];
                         if (bx0 > mx[0] || mn[0] > bx1) continue;
+                        if (exact) {
+                            const int64_t idx[3] = {x, y, z};
+                            if (tri_cell_separated(V, T, t, lo, ce, idx, eps)) continue;
+                        }
                         o.push_back(Pair{(uint32_t)((z * dm[1] + y) * dm[0] + x), t});
                     }
                 }
@@ -458,8 +521,30 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     }
     S->n_cells = S->dims[0] * S->dims[1] * S->dims[2];
     pairs_to_csr(pairs, S->n_cells, S->cell_start, S->cell_tri, S->max_per_cell);
-    if (grid->kind == DSIM_GRID_AABB)
-        compute_clearance(S->dims, S->lo, S->cell, S->cell_start, S->cell_tri, box, S->clearance);
+    if (grid->kind == DSIM_GRID_AABB) {
+        compute_clearance(S->dims, S->lo, S->cell, S->cell_start, S->cell_tri,
+                          [&](uint32_t t, const double b0[3], const double b1[3]) {
+                              const float4 a = box[(size_t)t * 2], b = box[(size_t)t * 2 + 1];
+                              double lo3[3] = {a.x, a.y, a.z}, hi3[3] = {b.x, b.y, b.z}, d2 = 0;
+                              for (int q = 0; q < 3; ++q) {
+                                  double g = std::max({0.0, lo3[q] - b1[q], b0[q] - hi3[q]});
+                                  d2 += g * g;
+                              }
+                              return d2;
+                          },
+                          S->clearance);
+    } else if (grid->kind == DSIM_GRID_TRIBOX) {
+        compute_clearance(S->dims, S->lo, S->cell, S->cell_start, S->cell_tri,
+                          [&](uint32_t t, const double b0[3], const double b1[3]) {
+                              double a[3], b[3], c[3];
+                              tri_vertices(V, T, t, a, b, c);
+                              dsim_host::TriBoxAxes X;
+                              dsim_host::tribox_setup(a, b, c, b0, b1, X);
+                              return dsim_host::tribox_gap2(X);
+                          },
+                          S->clearance);
+    }
+    S->label_axis = dsim_host::label_axis_of(mesh);
 
     if (!have_device) {  // host-only substrate: grid and helpers are inspectable, no sims
         *out = S;
@@ -478,8 +563,11 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
     // prefilter of a cell's candidates reads contiguous memory), and the
     // triangle's Moller-Trumbore vertices with its index in v0.w, so the test
     // needs no second lookup through cell_tri.
-    std::vector<float4> cp(ct.size()), cm(ct.size() * 3);
-    for (size_t k = 0; k < ct.size(); ++k) {
+    // Exact-membership grids list long triangles in many cells: their
+    // entries keep only the index and the step reads the triangle's record.
+    const bool cell_major = grid->kind != DSIM_GRID_TRIBOX;
+    std::vector<float4> cp(cell_major ? ct.size() : 0), cm(cell_major ? ct.size() * 3 : 0);
+    for (size_t k = 0; k < cp.size(); ++k) {
         if (S->cell_tri.empty()) break;
         const float4 *r = &mt[(size_t)ct[k] * 4];
         cp[k] = r[0];
@@ -487,8 +575,8 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
         cm[k * 3 + 1] = r[2];
         cm[k * 3 + 2] = r[3];
     }
-    if ((rc = upload(S->d_cell_tri, ct)) || (rc = upload(S->cell_plane, cp)) ||
-        (rc = upload(S->cell_mt, cm))) {
+    if ((rc = upload(S->d_cell_tri, ct)) || (cell_major && (rc = upload(S->cell_plane, cp))) ||
+        (cell_major && (rc = upload(S->cell_mt, cm)))) {
         delete S;
         return rc;
     }
@@ -549,6 +637,12 @@ extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helper
         return DSIM_OK;
     }
     if (!s->helpers.empty()) memcpy(helpers, s->helpers.data(), s->helpers.size() * 4);
+    return DSIM_OK;
+}
+
+extern "C" int dsim_substrate_label_axis(const dsim_substrate *s, uint32_t *axis) {
+    if (!s || !axis) return fail(DSIM_ERR_ARG, "NULL argument");
+    *axis = s->label_axis;
     return DSIM_OK;
 }
 

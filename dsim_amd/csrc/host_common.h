@@ -18,6 +18,8 @@ int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 
 // Checks the mesh arrays of dsim_substrate_create (indices, membranes, fibres).
 int check_mesh(const dsim_mesh *mesh);
+// Axis (0 x, 1 y, 2 z) of the physical mode's labelling rays for a mesh.
+uint32_t label_axis_of(const dsim_mesh *mesh);
 
 // The largest float whose square is at most x >= 0 (the float floor of the
 // square root, whatever the rounding of the double square root: the square of
@@ -83,6 +85,7 @@ struct dsim_substrate {
     std::vector<uint32_t> cell_start, cell_tri;
     uint32_t n_cells = 0, max_per_cell = 0;
     uint32_t kind = 0;
+    uint32_t label_axis = 0;  // ray axis of the physical-mode labeller (0 x, 1 y, 2 z)
     uint32_t dims[3] = {0, 0, 0};
     float lo[3] = {0, 0, 0}, cell[3] = {0, 0, 0}, inv_cell[3] = {0, 0, 0};
     std::vector<float> clearance;  // per cell: lower bound of the distance to any triangle

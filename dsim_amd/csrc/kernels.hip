@@ -295,6 +295,28 @@ __device__ __forceinline__ bool better(const Hit &c, const Hit &b) {
     return (c.d < b.d) || (c.d == b.d && c.idx < b.idx);
 }
 
+// Moller-Trumbore vertices of grid entry k and its triangle index: from the
+// cell-major copy, or (exact-membership grids keep none) from the
+// triangle's own record through cell_tri.  The branch is uniform.
+__device__ __forceinline__ uint32_t entry_mt(const StepParams &P, uint32_t k, float4 &v0,
+                                             float4 &e1, float4 &e2) {
+    if (P.cell_mt) {
+        v0 = P.cell_mt[(size_t)k * 3 + 0];
+        e1 = P.cell_mt[(size_t)k * 3 + 1];
+        e2 = P.cell_mt[(size_t)k * 3 + 2];
+        return __float_as_uint(v0.w);
+    }
+    const uint32_t t = P.cell_tri[k];
+    v0 = P.tri_mt[(size_t)t * 4 + 1];
+    e1 = P.tri_mt[(size_t)t * 4 + 2];
+    e2 = P.tri_mt[(size_t)t * 4 + 3];
+    return t;
+}
+// Plane (n, n.v0) of grid entry k's triangle.
+__device__ __forceinline__ float4 entry_plane(const StepParams &P, uint32_t k) {
+    return P.cell_plane ? P.cell_plane[k] : P.tri_mt[(size_t)P.cell_tri[k] * 4];
+}
+
 // cubeCollDetect, spinKernelCpu.cpp:311-337, over one CSR cell.
 template <int MODE>
 __device__ __forceinline__ void cell_test(const StepParams &P, uint32_t cell, uint32_t excluded,
@@ -311,7 +333,7 @@ __device__ __forceinline__ void cell_test(const StepParams &P, uint32_t cell, ui
         } else {
             // the cell-major plane record rejects most candidates without
             // touching the triangle record
-            if (!crosses_plane(P.cell_plane[k], ox, oy, oz, px, py, pz)) continue;
+            if (!crosses_plane(entry_plane(P, k), ox, oy, oz, px, py, pz)) continue;
             uint32_t t = P.cell_tri[k];
             if (t == excluded) continue;
             Hit c;
@@ -650,10 +672,8 @@ __device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, 
     if (q >= qlen) return false;
     const uint2 e = L.queue[q];
     owner = e.y;
-    const float4 v0 = P.cell_mt[(size_t)e.x * 3 + 0];
-    const float4 e1 = P.cell_mt[(size_t)e.x * 3 + 1];
-    const float4 e2 = P.cell_mt[(size_t)e.x * 3 + 2];
-    const uint32_t t = __float_as_uint(v0.w);
+    float4 v0, e1, e2;
+    const uint32_t t = entry_mt(P, e.x, v0, e1, e2);
     const float4 so = L.o[owner], sp = L.p[owner];
     if (t == __float_as_uint(so.w)) return false;
     if (mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) return true;
@@ -678,10 +698,8 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
         const uint32_t q = qb + lane;
         if (q < qlen) {
             const uint2 e = L.queue[q];
-            const float4 v0 = P.cell_mt[(size_t)e.x * 3 + 0];
-            const float4 e1 = P.cell_mt[(size_t)e.x * 3 + 1];
-            const float4 e2 = P.cell_mt[(size_t)e.x * 3 + 2];
-            const uint32_t t = __float_as_uint(v0.w);
+            float4 v0, e1, e2;
+            const uint32_t t = entry_mt(P, e.x, v0, e1, e2);
             const float4 so = L.o[e.y], sp = L.p[e.y];
             if (t != __float_as_uint(so.w) &&
                 mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) {
@@ -741,7 +759,7 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
         k = 0;
     }
     const float4 so = L.o[owner], sp = L.p[owner];
-    return crosses_plane(P.cell_plane[k], so.x, so.y, so.z, sp.x, sp.y, sp.z);
+    return crosses_plane(entry_plane(P, k), so.x, so.y, so.z, sp.x, sp.y, sp.z);
 }
 
 // One search round, called by all 64 lanes of the wave together.  For each
@@ -868,6 +886,22 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t run) {
     return ((k / run) * kXcds + x) * run + k % run;
 }
 
+// Periodic voxel: one face crossing per step at most (a step is far shorter
+// than the voxel); k counts the crossings, +1 for each wrap back from hi.
+__device__ __forceinline__ void wrap_axis(float &p, float lo, float hi, float len, int &k) {
+    if (p >= hi) { p -= len; k += 1; } else if (p < lo) { p += len; k -= 1; }
+}
+// Three signed 10-bit crossing counts in one word (the count wraps at +-512,
+// far beyond any walk of a voxel wider than a few steps).
+__device__ __forceinline__ void unpack_wrap(uint32_t w, int &kx, int &ky, int &kz) {
+    kx = (int)(w << 22) >> 22;
+    ky = (int)(w << 12) >> 22;
+    kz = (int)(w << 2) >> 22;
+}
+__device__ __forceinline__ uint32_t pack_wrap(int kx, int ky, int kz) {
+    return ((uint32_t)kx & 1023u) | (((uint32_t)ky & 1023u) << 10) | (((uint32_t)kz & 1023u) << 20);
+}
+
 // cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode: `iters` steps
 // for the blockDim.x spins of block `block` of the simulation described by P.
 template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
@@ -881,11 +915,14 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     float x = 0.f, y = 0.f, z = 0.f, mag = 0.f, ph = 0.f;
     float dx = 0.f, dy = 0.f, dz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     float safe = -1.0f;
+    // voxel crossings (periodic phase): the unwrapped position is x + kx * period
+    int kx = 0, ky = 0, kz = 0;
     if (live) {
         sid = P.spin_id ? P.spin_id[i] : i;
         x = P.x[i]; y = P.y[i]; z = P.z[i];
         mag = P.mag[i];
         if (!MOMENT) ph = P.phase[i];
+        if (PERIODIC && !MOMENT) unpack_wrap(P.wrap[i], kx, ky, kz);
         const uint32_t lab = P.label[i];
         comp = lab & 0xFFu; fiber = (lab >> 8) & 0xFFFFu;
         if (RNG == kRngMwc) { s0 = P.seed[2 * i]; s1 = P.seed[2 * i + 1]; }
@@ -987,9 +1024,9 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             }
             if (PERIODIC) {
                 const float wx = px, wy = py, wz = pz;
-                if (px >= 1.0f) px -= 2.0f; else if (px < -1.0f) px += 2.0f;
-                if (py >= 1.0f) py -= 2.0f; else if (py < -1.0f) py += 2.0f;
-                if (pz >= 1.0f) pz -= 2.0f; else if (pz < -1.0f) pz += 2.0f;
+                wrap_axis(px, P.wrap_lo[0], P.wrap_hi[0], P.wrap_len[0], kx);
+                wrap_axis(py, P.wrap_lo[1], P.wrap_hi[1], P.wrap_len[1], ky);
+                wrap_axis(pz, P.wrap_lo[2], P.wrap_hi[2], P.wrap_len[2], kz);
                 refresh |= (px != wx) | (py != wy) | (pz != wz);
             }
             x = px; y = py; z = pz;
@@ -997,7 +1034,16 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             const float dc = comp == 0 ? P.decay[0] : comp == 1 ? P.decay[1]
                            : comp == 2 ? P.decay[2] : P.decay[3];
             mag = fmaf(-mag, dc, mag);
-            if (!MOMENT) ph += (P.gx * x + P.gy * y + P.gz * z) * P.pc;
+            // spinKernelCpu.cpp:490 with the post-step position; in a periodic
+            // voxel that is the unwrapped position (x + k * period, exactly x
+            // for a spin that has not crossed a face)
+            if (!MOMENT && !PERIODIC) ph += (P.gx * x + P.gy * y + P.gz * z) * P.pc;
+            if (!MOMENT && PERIODIC) {
+                const float ux = x + (float)kx * P.wrap_len[0];
+                const float uy = y + (float)ky * P.wrap_len[1];
+                const float uz = z + (float)kz * P.wrap_len[2];
+                ph += (P.gx * ux + P.gy * uy + P.gz * uz) * P.pc;
+            }
             if (MOMENT && P.weight != 0.0f) {
                 float yv, tv;
                 yv = fmaf(P.weight, dx, -cx); tv = mx + yv; cx = (tv - mx) - yv; mx = tv;
@@ -1026,6 +1072,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         P.x[i] = x; P.y[i] = y; P.z[i] = z;
         P.mag[i] = mag;
         if (!MOMENT) P.phase[i] = ph;
+        if (PERIODIC && !MOMENT) P.wrap[i] = pack_wrap(kx, ky, kz);
         P.label[i] = comp | (fiber << 8);
         if (RNG == kRngMwc) { P.seed[2 * i] = s0; P.seed[2 * i + 1] = s1; }
         if (MOMENT) {
@@ -1468,15 +1515,20 @@ constexpr uint32_t kRetryPerBlockMax = 512;
 
 __device__ __forceinline__ uint32_t fiber_hash(uint32_t f) { return f * 0x9E3779B1u + 0x7F4A7C15u; }
 
-// The end point of spin i's ray on try `tries`: to 1.1 * max along +x (the
-// physical mode) or along the spin's dominant axis (findSpinsInFibers), then
-// for re-casts moved by a Philox jitter of up to 0.05 per axis.
+// The end point of spin i's ray on try `tries`: to 1.1 * max along the
+// substrate's ray axis (the physical mode) or along the spin's dominant axis
+// (findSpinsInFibers), then for re-casts moved by a Philox jitter of up to
+// 0.05 per axis.
 __device__ __forceinline__ void label_ray(const LabelParams &P, uint32_t i, uint32_t tries,
                                           float sx, float sy, float sz, float &ex, float &ey,
                                           float &ez) {
     ex = sx; ey = sy; ez = sz;
-    if (P.plus_x) {
+    if (P.ray_axis == 1) {
         ex = (float)(1.1 * (double)P.xmax);
+    } else if (P.ray_axis == 2) {
+        ey = (float)(1.1 * (double)P.ymax);
+    } else if (P.ray_axis == 3) {
+        ez = (float)(1.1 * (double)P.zmax);
     } else if ((fabsf(sx) > fabsf(sy)) & (fabsf(sx) > fabsf(sz))) {
         ex = (float)(((sx >= 0) - (sx < 0)) * 1.1 * (double)P.xmax);
     } else if (fabsf(sy) > fabsf(sz)) {
@@ -1596,10 +1648,12 @@ hipError_t launch_label(const LabelParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Does a triangle's box start past the grid's +x face?  No grid row lists
-// such a triangle in a cell that a ray from inside the grid box can reach.
+// Does a triangle's box start past the grid's face on the ray's side?  No
+// grid row lists such a triangle in a cell that a ray from inside the grid
+// box can reach.
 __device__ __forceinline__ bool past_grid(const LabelParams &P, float4 lo) {
-    return lo.x > P.hi[0];
+    const int a = (int)P.ray_axis - 1;
+    return (a == 0 ? lo.x : a == 1 ? lo.y : lo.z) > P.hi[a];
 }
 
 __global__ __launch_bounds__(256) void k_label_outside(LabelParams P) {
@@ -1632,41 +1686,53 @@ __device__ __forceinline__ bool listed(const LabelParams &P, uint32_t c, uint32_
 // Lanes that share one spin's row walk (consecutive lanes of a wave).
 constexpr uint32_t kRowLanes = 8;
 
-// The first +x ray of each spin (k_label's try 0): candidates are the
-// triangles listed in the cells of the ray's grid row from the spin's cell on,
-// each considered once (in the first of those cells that lists it), plus the
-// triangles whose boxes start past the grid's +x face.  k_label's candidates
-// are the triangles whose boxes hold (x, sy, sz) for some x on the ray; for
-// one that does not start past the grid, x = max(box lo.x, sx) lies in the
-// grid box, so the cell of the row holding x meets the triangle's box and
-// lists it (the grid lists a triangle in every cell whose closed membership
-// box meets its bounding box, and the cells' boxes tile the grid box).  The
-// kRowLanes lanes of a spin take every kRowLanes-th cell and outside triangle
-// and combine their sums; the sums do not depend on the order, so the result
-// equals k_label's first try.  Spins with an ambiguous ray, outside the grid
-// box, or beyond the ray's end are queued for the re-cast kernels.
+// The first ray of each spin along the substrate's ray axis A (k_label's
+// try 0): candidates are the triangles listed in the cells of the ray's grid
+// row from the spin's cell on, each considered once (in the first of those
+// cells that lists it), plus the triangles whose boxes start past the grid's
+// face on the ray's side.  For an AABB grid these are k_label's candidates:
+// those are the triangles whose boxes hold a point of the ray; for one that
+// does not start past the grid, the point of the ray at max(box lo_A, s_A)
+// lies in the grid box, so the cell of the row holding it meets the
+// triangle's box and lists it (the grid lists a triangle in every cell whose
+// closed membership box meets its bounding box, and the cells' boxes tile the
+// grid box).  An exact-membership grid lists a crossed triangle in the cell
+// of the row that holds the crossing, and in a contiguous run of the row's
+// cells (a convex triangle meets a convex row of boxes in an interval), so
+// every crossing is counted once.  The kRowLanes lanes of a spin take every
+// kRowLanes-th cell and outside triangle and combine their sums; the sums do
+// not depend on the order, so the result equals k_label's first try.  Spins
+// with an ambiguous ray, outside the grid box, or beyond the ray's end are
+// queued for the re-cast kernels.
 __global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
     const uint32_t i = gid / kRowLanes, g = gid % kRowLanes;
     const bool live = i < P.n;  // every lane stays for the shuffles below
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    if (live) { sx = P.x[i]; sy = P.y[i]; sz = P.z[i]; }
-    const float ex = (float)(1.1 * (double)P.xmax), ey = sy, ez = sz;
-    // the row walk covers rays that start in the grid box and run towards +x
+    const int A = (int)P.ray_axis - 1;
+    float s3[3] = {0.f, 0.f, 0.f};
+    if (live) { s3[0] = P.x[i]; s3[1] = P.y[i]; s3[2] = P.z[i]; }
+    const float sx = s3[0], sy = s3[1], sz = s3[2];
+    float ex, ey, ez;
+    label_ray(P, i, 0u, sx, sy, sz, ex, ey, ez);
+    const float eA = A == 0 ? ex : A == 1 ? ey : ez;
+    // the row walk covers rays that start in the grid box and run towards +A
     const bool row = live && sx >= P.lo[0] && sx <= P.hi[0] && sy >= P.lo[1] &&
-                     sy <= P.hi[1] && sz >= P.lo[2] && sz <= P.hi[2] && sx <= ex;
+                     sy <= P.hi[1] && sz >= P.lo[2] && sz <= P.hi[2] && s3[A] <= eA;
     RaySums R = {};
     if (row) {
-        const uint32_t cx0 = label_cell(P, sx, 0), cy = label_cell(P, sy, 1),
-                       cz = label_cell(P, sz, 2);
-        const uint32_t rowc = (cz * P.dims[1] + cy) * P.dims[0];
-        for (uint32_t cx = cx0 + g; cx < P.dims[0]; cx += kRowLanes) {
-            const uint32_t c = rowc + cx;
+        uint32_t c3[3] = {label_cell(P, sx, 0), label_cell(P, sy, 1), label_cell(P, sz, 2)};
+        const uint32_t stride[3] = {1u, P.dims[0], P.dims[0] * P.dims[1]};
+        const uint32_t cA0 = c3[A];
+        c3[A] = 0;
+        const uint32_t rowc = (c3[2] * P.dims[1] + c3[1]) * P.dims[0] + c3[0];
+        const uint32_t sA = stride[A];
+        for (uint32_t cA = cA0 + g; cA < P.dims[A]; cA += kRowLanes) {
+            const uint32_t c = rowc + cA * sA;
             for (uint32_t k = P.cell_start[c]; k < P.cell_start[c + 1]; ++k) {
                 const uint32_t t = P.cell_tri[k];
                 const float4 lo = P.tri_aabb[2 * t], hi = P.tri_aabb[2 * t + 1];
-                if (past_grid(P, lo)) continue;                 // in the outside list
-                if (cx > cx0 && listed(P, c - 1, t)) continue;  // considered one cell back
+                if (past_grid(P, lo)) continue;                  // in the outside list
+                if (cA > cA0 && listed(P, c - sA, t)) continue;  // considered one cell back
                 ray_add(P, t, lo, hi, sx, sy, sz, ex, ey, ez, R);
             }
         }
@@ -1828,29 +1894,34 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
 }
 
 // resetSpins position draw (spinSystem.cpp:1386-1398) from Philox stream 2.
+// Per axis: width * (u - 0.5) + centre, u in (0, 1] (the box [-b, b] has
+// width 2b and centre 0, which leaves resetSpins' expression unchanged).
 __global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, float *y, float *z,
                                                       float *mag, float *phase, uint32_t *label,
-                                                      uint32_t *id, float box, uint32_t k0,
-                                                      uint32_t k1) {
+                                                      uint32_t *id, uint32_t *wrap, float3 width,
+                                                      float3 centre, uint32_t k0, uint32_t k1) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     id[i] = i;
     uint4 o = philox(make_uint4(i, 0u, 0u, 2u), k0, k1);
-    float b2 = box * 2.0f;
-    x[i] = b2 * (u32_unit(o.x) - 0.5f);
-    y[i] = b2 * (u32_unit(o.y) - 0.5f);
-    z[i] = b2 * (u32_unit(o.z) - 0.5f);
+    x[i] = width.x * (u32_unit(o.x) - 0.5f) + centre.x;
+    y[i] = width.y * (u32_unit(o.y) - 0.5f) + centre.y;
+    z[i] = width.z * (u32_unit(o.z) - 0.5f) + centre.z;
     mag[i] = 1.0f;
     phase[i] = 0.0f;
     label[i] = kFiberNone << 8;
+    wrap[i] = 0u;
 }
 
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
-                               uint32_t *label, uint32_t *id, float start_box, uint32_t key0,
-                               uint32_t key1, hipStream_t s) {
+                               uint32_t *label, uint32_t *id, uint32_t *wrap, const float lo[3],
+                               const float hi[3], uint32_t key0, uint32_t key1, hipStream_t s) {
     if (n == 0) return hipSuccess;
+    const float3 w = make_float3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+    const float3 c = make_float3(0.5f * (hi[0] + lo[0]), 0.5f * (hi[1] + lo[1]),
+                                 0.5f * (hi[2] + lo[2]));
     hipLaunchKernelGGL(k_init_uniform, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, mag,
-                       phase, label, id, start_box, key0, key1);
+                       phase, label, id, wrap, w, c, key0, key1);
     return hipGetLastError();
 }
 

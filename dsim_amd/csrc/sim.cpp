@@ -26,8 +26,14 @@ enum Field {
     F_X, F_Y, F_Z, F_MAG, F_PHASE, F_LABEL, F_ID,
     F_DISP, F_MOM = F_DISP + 3, F_MOMC = F_MOM + 3, F_X0 = F_MOMC + 3,
     F_HITS = F_X0 + 3,  // membrane hits since the last reorder (orders the next one)
-    F_COUNT = F_HITS + 1
+    F_WRAP,             // periodic voxel crossings (phase of the unwrapped position)
+    F_COUNT
 };
+
+// Finished step-launch timings are folded into ms_done once this many event
+// pairs are outstanding, so a long run of integrate calls holds a bounded
+// number of events.
+const size_t kMaxEventPairs = 64;
 
 const uint32_t kDefaultStepsPerLaunch = 1024;
 
@@ -88,6 +94,14 @@ extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32
     if (!sub->on_device) return fail(DSIM_ERR_NODEV, "substrate was created without a HIP device");
     if (p->mode > 1 || p->boundary > 1 || (p->rng != DSIM_RNG_MWC && p->rng != DSIM_RNG_PHILOX))
         return fail(DSIM_ERR_ARG, "bad mode/boundary/rng");
+    bool voxel_set = false;
+    for (int k = 0; k < 3; ++k) voxel_set |= p->voxel_lo[k] != 0.0f || p->voxel_hi[k] != 0.0f;
+    if (voxel_set)
+        for (int k = 0; k < 3; ++k)
+            if (!(p->voxel_hi[k] > p->voxel_lo[k]) || !std::isfinite(p->voxel_lo[k]) ||
+                !std::isfinite(p->voxel_hi[k]))
+                return fail(DSIM_ERR_ARG, "voxel axis %d: [%g, %g) is empty", k, p->voxel_lo[k],
+                            p->voxel_hi[k]);
     if (p->mode == DSIM_MODE_REFERENCE && p->boundary != DSIM_BOUNDARY_CLAMP)
         return fail(DSIM_ERR_ARG, "the reference mode clamps (spinKernelCpu.cpp:469-474)");
     if (!(p->dt > 0.0f) || !(p->permeability >= 0.0f && p->permeability <= 1.0f))
@@ -97,6 +111,8 @@ extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32
     S->sub = sub;
     sub->refs++;
     S->prm = *p;
+    if (!voxel_set)
+        for (int k = 0; k < 3; ++k) { S->prm.voxel_lo[k] = -1.0f; S->prm.voxel_hi[k] = 1.0f; }
     if (S->prm.steps_per_launch == 0) S->prm.steps_per_launch = kDefaultStepsPerLaunch;
     S->n = n;
     hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
@@ -211,6 +227,7 @@ extern "C" int dsim_set_state_aos(dsim_sim *s, const float *pos, const dsim_spin
         HIP_TRY(hipMemcpy(s->ff(F_X), a.data(), n * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->ff(F_Y), b.data(), n * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->ff(F_Z), c.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(s->f(F_WRAP), 0, (size_t)n * 4));  // new positions: no crossings yet
     }
     if (spins) {
         std::vector<float> m(n), ph(n);
@@ -314,6 +331,11 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.reflection_type = s->prm.reflection_type;
     P.clamp_hi = 1.0f - s->prm.spin_radius;
     P.clamp_lo = -1.0f + s->prm.spin_radius;
+    for (int k = 0; k < 3; ++k) {
+        P.wrap_lo[k] = s->prm.voxel_lo[k];
+        P.wrap_hi[k] = s->prm.voxel_hi[k];
+        P.wrap_len[k] = s->prm.voxel_hi[k] - s->prm.voxel_lo[k];
+    }
     P.n = s->n;
     P.x = s->ff(F_X); P.y = s->ff(F_Y); P.z = s->ff(F_Z);
     P.mag = s->ff(F_MAG); P.phase = s->ff(F_PHASE);
@@ -322,6 +344,7 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.seed = s->seed.p;
     P.disp = s->ff(F_DISP); P.mom = s->ff(F_MOM); P.momc = s->ff(F_MOMC);
     P.hits = s->sorting() ? s->f(F_HITS) : nullptr;
+    P.wrap = s->f(F_WRAP);
     P.key0 = (uint32_t)s->prm.seed;
     P.key1 = (uint32_t)(s->prm.seed >> 32);
     P.flags = s->flags.p;
@@ -349,6 +372,23 @@ static int spatial_sort(dsim_sim *s) {
     s->permuted = true;
     s->steps_since_sort = 0;
     s->sorts++;
+    return DSIM_OK;
+}
+
+// Adds the elapsed times of the oldest `count` event pairs to ms_done and
+// destroys them (waiting for the last of them to complete).
+static int fold_events(dsim_sim *s, size_t count) {
+    count = std::min(count, s->ev.size());
+    if (count == 0) return DSIM_OK;
+    HIP_TRY(hipEventSynchronize(s->ev[count - 1].second));
+    for (size_t k = 0; k < count; ++k) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, s->ev[k].first, s->ev[k].second));
+        s->ms_done += t;
+        (void)hipEventDestroy(s->ev[k].first);
+        (void)hipEventDestroy(s->ev[k].second);
+    }
+    s->ev.erase(s->ev.begin(), s->ev.begin() + (std::ptrdiff_t)count);
     return DSIM_OK;
 }
 
@@ -391,16 +431,22 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
         // each step launch is bracketed by its own events: the timing covers
         // the step kernel only, not the reorders between launches
-        hipEvent_t e0, e1;
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, s->stream));
-        hipError_t e = dsim::launch_step(P, k, (int)s->prm.mode,
-                                         s->prm.boundary == DSIM_BOUNDARY_PERIODIC, rng,
-                                         trace != nullptr, moment, s->stream);
+        if (s->ev.size() >= kMaxEventPairs)
+            if (int rc = fold_events(s, s->ev.size() / 2)) return rc;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+        if (e != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            return fail(DSIM_ERR_HIP, "step events: %s", hipGetErrorString(e));
+        }
+        s->ev.emplace_back(e0, e1);  // owned by the handle from here on
+        e = dsim::launch_step(P, k, (int)s->prm.mode, s->prm.boundary == DSIM_BOUNDARY_PERIODIC,
+                              rng, trace != nullptr, moment, s->stream);
         if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
         HIP_TRY(hipEventRecord(e1, s->stream));
-        s->ev.emplace_back(e0, e1);
         done += k;
         s->steps_since_sort += k;
         s->launches++;
@@ -534,11 +580,20 @@ extern "C" int dsim_get_moment(dsim_sim *s, float *moment, float *x0) {
 }
 
 extern "C" int dsim_init_spins_uniform(dsim_sim *s, float box) {
+    const float lo[3] = {-box, -box, -box}, hi[3] = {box, box, box};
+    return dsim_init_spins_box(s, lo, hi);
+}
+
+extern "C" int dsim_init_spins_box(dsim_sim *s, const float lo[3], const float hi[3]) {
     if (int rc = check_sim(s)) return rc;
+    if (!lo || !hi) return fail(DSIM_ERR_ARG, "NULL box");
+    for (int k = 0; k < 3; ++k)
+        if (!(hi[k] >= lo[k]) || !std::isfinite(lo[k]) || !std::isfinite(hi[k]))
+            return fail(DSIM_ERR_ARG, "box axis %d: [%g, %g]", k, lo[k], hi[k]);
     hipError_t e = dsim::launch_init_uniform(s->n, s->ff(F_X), s->ff(F_Y), s->ff(F_Z), s->ff(F_MAG),
-                                             s->ff(F_PHASE), s->f(F_LABEL), s->f(F_ID), box,
-                                             (uint32_t)s->prm.seed, (uint32_t)(s->prm.seed >> 32),
-                                             s->stream);
+                                             s->ff(F_PHASE), s->f(F_LABEL), s->f(F_ID),
+                                             s->f(F_WRAP), lo, hi, (uint32_t)s->prm.seed,
+                                             (uint32_t)(s->prm.seed >> 32), s->stream);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
     s->permuted = false;
     s->moment_ready = false;
@@ -561,11 +616,16 @@ extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
     L.tries = s->tries.p;
     L.xmax = xmax; L.ymax = ymax; L.zmax = zmax;
     L.max_tries = max_tries;
-    L.plus_x = s->prm.mode == DSIM_MODE_PHYSICAL ? 1u : 0u;
-    // +x rays over an AABB grid take the grid-row labeller first; the rest,
-    // and every re-cast, go through the tiled pass over all triangles
+    // the physical mode casts along the substrate's ray axis (+x unless the
+    // mesh's long walls run along x), the reference mode along the spin's
+    // dominant axis
+    L.ray_axis = s->prm.mode == DSIM_MODE_PHYSICAL ? s->sub->label_axis + 1u : 0u;
+    // axis rays over an AABB or exact-membership grid take the grid-row
+    // labeller first; the rest, and every re-cast, go through the tiled pass
+    // over all triangles
     const dsim_substrate *sub = s->sub;
-    const bool rows = L.plus_x && sub->kind == DSIM_GRID_AABB && sub->n_cells > 0 && s->n > 0;
+    const bool rows = L.ray_axis && sub->kind != DSIM_GRID_REFERENCE && sub->n_cells > 0 &&
+                      s->n > 0;
     dsim_host::DevBuf<uint32_t> work;  // retry queue, counters, outside list
     hipError_t e;
     if (rows) {
@@ -619,14 +679,7 @@ extern "C" int dsim_sync(dsim_sim *s) {
 extern "C" int dsim_step_timing(dsim_sim *s, double *ms, uint64_t *launches) {
     if (int rc = check_sim(s)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
-    for (auto &e : s->ev) {
-        float t = 0.f;
-        HIP_TRY(hipEventElapsedTime(&t, e.first, e.second));
-        s->ms_done += t;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    s->ev.clear();
+    if (int rc = fold_events(s, s->ev.size())) return rc;
     if (ms) *ms = s->ms_done;
     if (launches) *launches = s->launches;
     return DSIM_OK;

@@ -59,7 +59,8 @@ class Params(C.Structure):
                 ("permeability", C.c_float), ("reflection_type", C.c_uint32), ("mode", C.c_uint32),
                 ("boundary", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint64),
                 ("spin_radius", C.c_float), ("sort_interval", C.c_uint32),
-                ("steps_per_launch", C.c_uint32)]
+                ("steps_per_launch", C.c_uint32), ("voxel_lo", C.c_float * 3),
+                ("voxel_hi", C.c_float * 3)]
 
 
 _lib = None
@@ -105,6 +106,7 @@ def lib():
             "dsim_set_step": [_vp, C.c_uint64],
             "dsim_get_step": [_vp, C.POINTER(C.c_uint64)],
             "dsim_init_spins_uniform": [_vp, C.c_float],
+            "dsim_init_spins_box": [_vp, _fp, _fp],
             "dsim_label_spins": [_vp, C.c_float, C.c_float, C.c_float, C.c_uint32, _up],
             "dsim_integrate": [_vp, _fp, C.c_float, C.c_uint32],
             "dsim_integrate_injected": [_vp, _fp, C.c_float, _fp, C.c_uint32, _up],
@@ -215,7 +217,7 @@ class Simulation:
 
     def __init__(self, substrate: Substrate, n, *, dt, std_dev, t2=(np.inf,) * 4, permeability=0.0,
                  reflection_type=1, mode=MODE_REFERENCE, boundary=BOUNDARY_CLAMP, rng=RNG_PHILOX,
-                 seed=1, spin_radius=0.005, sort_interval=0, steps_per_launch=0):
+                 seed=1, spin_radius=0.005, sort_interval=0, steps_per_launch=0, voxel=None):
         self.substrate = substrate
         self.n = int(n)
         p = Params()
@@ -231,6 +233,9 @@ class Simulation:
         p.spin_radius = float(spin_radius)
         p.sort_interval = int(sort_interval)
         p.steps_per_launch = int(steps_per_launch)
+        if voxel is not None:  # ((lo x, y, z), (hi x, y, z)) of a non-cubic periodic voxel
+            p.voxel_lo = (C.c_float * 3)(*[float(v) for v in voxel[0]])
+            p.voxel_hi = (C.c_float * 3)(*[float(v) for v in voxel[1]])
         self.params = p
         h = _vp()
         check(lib().dsim_sim_create(substrate.h, C.byref(p), self.n, C.byref(h)))
@@ -281,6 +286,11 @@ class Simulation:
 
     def init_uniform(self, start_box):
         check(lib().dsim_init_spins_uniform(self.h, float(start_box)))
+
+    def init_box(self, lo, hi):
+        a = np.ascontiguousarray(lo, np.float32)
+        b = np.ascontiguousarray(hi, np.float32)
+        check(lib().dsim_init_spins_box(self.h, _p(a, C.c_float), _p(b, C.c_float)))
 
     def label(self, xmax=1.0, ymax=1.0, zmax=1.0, max_tries=20):
         amb = C.c_uint32()

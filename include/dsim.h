@@ -65,7 +65,10 @@ typedef struct {
 
 enum {
     DSIM_GRID_REFERENCE = 0, /* n^3 cubes over [-1,1]^3, reference membership (populateCubes) */
-    DSIM_GRID_AABB = 1       /* any box, triangle-AABB membership (conservative) */
+    DSIM_GRID_AABB = 1,      /* any box, triangle-AABB membership (conservative) */
+    DSIM_GRID_TRIBOX = 2     /* any box, exact triangle/box membership within eps (separating
+                              * axes): for meshes whose long triangles do not run along a
+                              * grid axis; entries keep only the triangle index */
 };
 
 typedef struct {
@@ -99,6 +102,8 @@ typedef struct {
                                * rate, then cell.  Results do not depend on it. */
     uint32_t steps_per_launch; /* steps fused per kernel launch (0 = 1024; with reordering at
                                   most sort_interval) */
+    float voxel_lo[3];        /* periodic voxel [voxel_lo, voxel_hi) per axis, normalised;     */
+    float voxel_hi[3];        /* all six zero means [-1, 1) on every axis (the cubic voxel)    */
 } dsim_params;
 
 typedef struct dsim_substrate dsim_substrate;
@@ -142,6 +147,12 @@ int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers);
  * detection while a spin that was inside the grid at its last check has
  * walked a path shorter than this clearance. */
 int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance);
+/* No reference counterpart: the axis of the physical mode's labelling rays
+ * (0 +x, 1 +y, 2 +z), chosen from the mesh so that the rays do not run along
+ * its long walls (findSpinsInFibers casts along the spin's dominant axis,
+ * spinSystem.cpp:1677-1683, which misses an open tube along that axis).  +x
+ * unless the triangles parallel to x hold more than 5% of the mesh's extent. */
+int dsim_substrate_label_axis(const dsim_substrate *s, uint32_t *axis);
 /* Device bytes held by the substrate. */
 int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes);
 
@@ -164,6 +175,10 @@ int dsim_get_step(dsim_sim *sim, uint64_t *step);
  * positions uniform in [-start_box, start_box]^3 from the Philox stream,
  * magnitude 1, phase 0, labels cleared.  Follow with dsim_label_spins. */
 int dsim_init_spins_uniform(dsim_sim *sim, float start_box);
+/* The same draw over a box [lo, hi] per axis (a non-cubic periodic voxel):
+ * coordinate = (hi - lo) * (u - 0.5) + (hi + lo) / 2, u from the same Philox
+ * stream, so the cube [-b, b]^3 gives dsim_init_spins_uniform(sim, b). */
+int dsim_init_spins_box(dsim_sim *sim, const float lo[3], const float hi[3]);
 
 /* findSpinsInFibers (spinSystem.cpp:1615-1852), on the device.  The reference
  * mode casts its ray along the spin's dominant axis to 1.1*max (as the
@@ -177,7 +192,9 @@ int dsim_label_spins(dsim_sim *sim, float xmax, float ymax, float zmax, uint32_t
 
 /* integrateSystem (spinSystem.cuh:34-45, spinSystem.cu:221-270): advance all
  * spins `iterations` steps under a constant gradient, phase += g.pos *
- * phaseConstant per step, state left on the device. */
+ * phaseConstant per step with pos the position after the step (in a periodic
+ * voxel the unwrapped position: pos + crossings * period per axis), state
+ * left on the device. */
 int dsim_integrate(dsim_sim *sim, const float grad_scaled[3], float phase_constant,
                    uint32_t iterations);
 /* Test entry: as dsim_integrate but the normals and uniform of every step come

@@ -66,12 +66,14 @@ class WorldC(C.Structure):
         ("spin_radius", C.c_float), ("boundary", C.c_uint32), ("test", C.c_uint32),
         ("traversal", C.c_uint32), ("transitions", C.c_uint32),
         ("fiber_has_myelin", _u8p), ("n_fibers", C.c_uint32),
+        ("wrap_lo", C.c_float * 3), ("wrap_hi", C.c_float * 3),
     ]
 
 
 class StateC(C.Structure):
     _fields_ = [("n", C.c_uint32), ("pos", _fp), ("mag", _fp), ("phase", _fp), ("comp", _u8p),
-                ("fiber", _u16p), ("seed", _up), ("disp", _fp), ("moment", _fp), ("moment_c", _fp)]
+                ("fiber", _u16p), ("seed", _up), ("disp", _fp), ("moment", _fp), ("moment_c", _fp),
+                ("wrap", C.POINTER(C.c_int32))]
 
 
 class StepArgsC(C.Structure):
@@ -250,7 +252,8 @@ class World:
     def __init__(self, vertices, trgls, tri_info, *, grid="reference", n_cubes=50, dims=None,
                  lo=None, cell=None, eps=None, std_dev=(0, 0, 0, 0), t2=(np.inf,) * 4, dt=0.005,
                  permeability=0.0, reflection_type=1, spin_radius=0.005, boundary=0, test=0,
-                 traversal=0, transitions=0, fiber_has_myelin=None, grid_arrays=None):
+                 traversal=0, transitions=0, fiber_has_myelin=None, grid_arrays=None,
+                 voxel_lo=(0.0, 0.0, 0.0), voxel_hi=(0.0, 0.0, 0.0)):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.trgls = np.ascontiguousarray(trgls, np.uint32).reshape(-1, 3)
         self.tri_info = np.ascontiguousarray(tri_info, np.uint32).reshape(-1, 3)
@@ -292,6 +295,8 @@ class World:
                            spin_radius=np.float32(spin_radius), boundary=boundary, test=test,
                            traversal=traversal, transitions=transitions)
         self.n_fibers = n_fibers
+        self.voxel_lo = tuple(float(np.float32(v)) for v in voxel_lo)
+        self.voxel_hi = tuple(float(np.float32(v)) for v in voxel_hi)
         self._build()
 
     def _build(self):
@@ -324,6 +329,8 @@ class World:
         w.transitions = int(p["transitions"])
         w.fiber_has_myelin = _p(self.fiber_has_myelin, C.c_uint8)
         w.n_fibers = self.n_fibers
+        w.wrap_lo = (C.c_float * 3)(*self.voxel_lo)
+        w.wrap_hi = (C.c_float * 3)(*self.voxel_hi)
         self.c = w
 
     def set(self, **kw):
@@ -364,7 +371,12 @@ class World:
         return (np.array([r.x, r.y, r.z], np.float32), c8.value, f16.value,
                 (h.first, h.last, h.ncoll))
 
-    def find_spins(self, pos, xmax=1.0, ymax=1.0, zmax=1.0, max_tries=20, plus_x=False):
+    def find_spins(self, pos, xmax=1.0, ymax=1.0, zmax=1.0, max_tries=20, plus_x=False,
+                   ray_axis=None):
+        """ray_axis 0/1/2 casts along +x/+y/+z (plus_x=True is ray_axis 0);
+        neither: the reference's dominant-axis ray."""
+        if ray_axis is not None:
+            plus_x = int(ray_axis) + 1
         P = np.ascontiguousarray(pos, np.float32).ravel()
         n = P.size // 3
         comp = np.zeros(n, np.uint8)
@@ -377,7 +389,8 @@ class World:
 
 
 class State:
-    def __init__(self, pos, comp=None, fiber=None, mag=None, phase=True, seed=None, moment=False):
+    def __init__(self, pos, comp=None, fiber=None, mag=None, phase=True, seed=None, moment=False,
+                 wrap=True):
         self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3).copy()
         n = len(self.pos)
         self.n = n
@@ -388,6 +401,8 @@ class State:
         self.fiber = (np.full(n, FIBER_NONE, np.uint16) if fiber is None
                       else np.asarray(fiber, np.uint16).copy())
         self.seed = None if seed is None else np.ascontiguousarray(seed, np.uint32).reshape(-1, 2).copy()
+        # periodic voxel crossings (wrap=False: the phase of the wrapped position)
+        self.wrap = np.zeros((n, 3), np.int32) if wrap else None
         if moment:
             self.disp = np.zeros((n, 3), np.float32)
             self.moment = np.zeros((n, 3), np.float32)
@@ -407,6 +422,7 @@ class State:
         s.disp = _p(self.disp, C.c_float)
         s.moment = _p(self.moment, C.c_float)
         s.moment_c = _p(self.moment_c, C.c_float)
+        s.wrap = _p(self.wrap, C.c_int32)
         return s
 
 

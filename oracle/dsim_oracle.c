@@ -769,19 +769,36 @@ static void step_one(const orc_world *w, orc_state *s, uint32_t i, uint32_t it, 
         s->disp[i * 3 + 1] += pos.y - o.y;
         s->disp[i * 3 + 2] += pos.z - o.z;
     }
+    int32_t kw[3] = {0, 0, 0};
+    if (s->wrap) { kw[0] = s->wrap[i * 3 + 0]; kw[1] = s->wrap[i * 3 + 1]; kw[2] = s->wrap[i * 3 + 2]; }
+    float wlo[3] = {-1.0f, -1.0f, -1.0f}, whi[3] = {1.0f, 1.0f, 1.0f};
+    if (w->wrap_lo[0] != 0.0f || w->wrap_hi[0] != 0.0f || w->wrap_lo[1] != 0.0f ||
+        w->wrap_hi[1] != 0.0f || w->wrap_lo[2] != 0.0f || w->wrap_hi[2] != 0.0f)
+        for (int k = 0; k < 3; ++k) { wlo[k] = w->wrap_lo[k]; whi[k] = w->wrap_hi[k]; }
+    float wlen[3] = {whi[0] - wlo[0], whi[1] - wlo[1], whi[2] - wlo[2]};
     if (w->boundary == 1) {
-        /* build-defined periodic wrap into [-1,1) */
-        if (pos.x >= 1.0f) pos.x -= 2.0f; else if (pos.x < -1.0f) pos.x += 2.0f;
-        if (pos.y >= 1.0f) pos.y -= 2.0f; else if (pos.y < -1.0f) pos.y += 2.0f;
-        if (pos.z >= 1.0f) pos.z -= 2.0f; else if (pos.z < -1.0f) pos.z += 2.0f;
+        /* build-defined periodic wrap into [lo, hi) per axis, counting crossings */
+        float *c[3] = {&pos.x, &pos.y, &pos.z};
+        for (int k = 0; k < 3; ++k) {
+            if (*c[k] >= whi[k]) { *c[k] -= wlen[k]; kw[k] += 1; }
+            else if (*c[k] < wlo[k]) { *c[k] += wlen[k]; kw[k] -= 1; }
+        }
+        if (s->wrap) { s->wrap[i * 3 + 0] = kw[0]; s->wrap[i * 3 + 1] = kw[1]; s->wrap[i * 3 + 2] = kw[2]; }
     }
     s->pos[i * 3 + 0] = pos.x; s->pos[i * 3 + 1] = pos.y; s->pos[i * 3 + 2] = pos.z;
     /* spinKernelCpu.cpp:486 forward-Euler T2; production uses dt/T2 */
     if (w->test == 0) s->mag[i] += -s->mag[i] / w->t2[comp] * w->dt;
     else s->mag[i] = fmaf(-s->mag[i], w->decay[comp], s->mag[i]);
-    /* spinKernelCpu.cpp:490 phase */
-    if (s->phase)
+    /* spinKernelCpu.cpp:490 phase, post-step position; in the periodic voxel
+     * (build-defined) the unwrapped position x + k * period */
+    if (s->phase && !(w->boundary == 1 && s->wrap))
         s->phase[i] += (a->grad.x * pos.x + a->grad.y * pos.y + a->grad.z * pos.z) * a->phase_constant;
+    else if (s->phase) {
+        float ux = pos.x + (float)kw[0] * wlen[0];
+        float uy = pos.y + (float)kw[1] * wlen[1];
+        float uz = pos.z + (float)kw[2] * wlen[2];
+        s->phase[i] += (a->grad.x * ux + a->grad.y * uy + a->grad.z * uz) * a->phase_constant;
+    }
     if (s->moment && a->weight != 0.0f) {
         for (int k = 0; k < 3; ++k) {
             float y = fmaf(a->weight, s->disp[i * 3 + k], -s->moment_c[i * 3 + k]);
@@ -860,8 +877,14 @@ uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n, float 
         uint16_t fib = ORC_FIBER_NONE;
         for (;;) {
             orc_f3 le = sp;
-            if (plus_x)  /* build-defined: ray along +x (open prisms parallel to z) */
+            /* build-defined: ray along +x, +y or +z (plus_x = 1, 2, 3), so that
+             * open prisms not parallel to that axis are counted */
+            if (plus_x == 1)
                 le.x = (float)(1.1 * (double)xmax);
+            else if (plus_x == 2)
+                le.y = (float)(1.1 * (double)ymax);
+            else if (plus_x == 3)
+                le.z = (float)(1.1 * (double)zmax);
             else if ((fabsf(sp.x) > fabsf(sp.y)) & (fabsf(sp.x) > fabsf(sp.z)))
                 le.x = (float)(((sp.x >= 0) - (sp.x < 0)) * 1.1 * (double)xmax);
             else if (fabsf(sp.y) > fabsf(sp.z))

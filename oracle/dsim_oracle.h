@@ -23,7 +23,10 @@
  *   orc_coll_detect_grid .... hand-built multi-hit cases; grid-independence of
  *                             the nearest-hit mode vs brute force over all tris
  *   orc_integrate ........... invariants (impermeable closed mesh keeps labels,
- *                             free diffusion variance, exp(-bD))
+ *                             free diffusion variance, T2 closed form); the
+ *                             per-step phase by a one-step known answer
+ *                             (post-step position) and by PGSE against
+ *                             exp(-bD) in the clamp box and the periodic voxel
  *   orc_find_spins .......... analytic point-in-prism on the sample meshes
  *   orc_mr_signal ........... closed form
  *   orc_parse_mesh .......... the reference's sample mesh files
@@ -84,6 +87,9 @@ typedef struct {
     uint32_t transitions;       /* 0 reference table, 1 physical table */
     const uint8_t *fiber_has_myelin;
     uint32_t n_fibers;
+    /* build-defined periodic voxel [wrap_lo, wrap_hi) per axis (boundary 1);
+     * all zero means [-1, 1) on every axis */
+    float wrap_lo[3], wrap_hi[3];
 } orc_world;
 
 /* Per-spin state, host layout of the reference: pos AoS xyz (m_hPos),
@@ -99,6 +105,8 @@ typedef struct {
     float *disp;        /* [n*3] unwrapped displacement (moment mode) or NULL */
     float *moment;      /* [n*3] sum of w*disp (moment mode) or NULL */
     float *moment_c;    /* [n*3] Kahan compensation or NULL */
+    int32_t *wrap;      /* [n*3] periodic voxel crossings (+1 per wrap back from hi), or
+                         * NULL; with it the phase uses the unwrapped position */
 } orc_state;
 
 enum { ORC_RNG_INJECT = 0, ORC_RNG_MWC = 1, ORC_RNG_PHILOX = 2 };
@@ -167,7 +175,8 @@ void orc_integrate(const orc_world *w, orc_state *s, uint32_t iters,
 
 /* ---- labelling / readout --------------------------------------------------- */
 /* findSpinsInFibers with a deterministic retry jitter.  Returns the number of
- * spins whose labels stayed ambiguous after max_tries. */
+ * spins whose labels stayed ambiguous after max_tries.  plus_x: 0 the
+ * reference's dominant-axis ray, 1/2/3 a ray along +x/+y/+z. */
 uint32_t orc_find_spins(const orc_world *w, const float *pos, uint32_t n,
                         float xmax, float ymax, float zmax, uint32_t max_tries,
                         uint32_t plus_x, uint8_t *comp, uint16_t *fiber, uint32_t *tries_out);

@@ -598,3 +598,75 @@ def test_pgse_schedule_closed_form():
     # R = 0: the B+L branch claims the readout update, so no readout happens
     _, (_, _, _, ro) = O.pgse_schedule(10.0, 40.0, 0.0, 0.5, 1)
     assert ro == 0
+
+
+def test_find_spins_plus_x_mt_matches_point_in_prism(ref_inputs):
+    # the physical mode's labeller: Moller-Trumbore crossings of a +x ray
+    m = O.parse_mesh(os.path.join(ref_inputs, "sim.fibers_1_r004_f09"), 3.0)
+    w = O.World(m["vertices"], m["trgls"], m["tri_info"], test=1, traversal=1, transitions=1)
+    rings = _rings(m)
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-0.95, 0.95, (3000, 3)).astype(np.float32)
+    comp, fib, tries, amb = w.find_spins(pos, m["xmax"], m["ymax"], m["zmax"], plus_x=True)
+    assert amb == 0
+    n_checked = 0
+    for i, (x, y, z) in enumerate(pos):
+        d = min(np.min(np.hypot(*(rings[k] - (x, y)).T)) for k in (0, 1))
+        if d < 1e-3:
+            continue
+        ina = _poly_contains(rings[0], x, y)
+        inm = _poly_contains(rings[1], x, y)
+        want = 1 if ina else (2 if inm else 0)
+        assert comp[i] == want, (i, x, y, z)
+        assert fib[i] == (0 if want else 65535)
+        n_checked += 1
+    assert n_checked > 2500
+    assert set(np.unique(comp)) == {0, 1, 2}
+
+
+def test_find_spins_plus_x_mt_on_a_packing():
+    # open 32-gon prisms of a periodic packing (with their images around the
+    # voxel): every spin away from a wall gets the fibre whose polygon holds it,
+    # and the intra-axonal fraction equals the polygons' area fraction
+    from dsim_amd import substrate as S
+    rng = np.random.default_rng(21)
+    ax = S.make_axons(40, 3.0, 2.0, rng, fraction=0.55, ngon=32)
+    V, T, I, s = S.prism_mesh(ax, ngon=32, z_range=(-1.2 * ax.voxel / 2, 1.2 * ax.voxel / 2),
+                              halo=0.5)
+    w = O.World(V, T, I, test=1, traversal=1, transitions=1, grid="aabb", dims=(1, 1, 1),
+                lo=(-2, -2, -2), cell=(4, 4, 4), eps=0.0)
+    n = 4000
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    xmax = float(np.abs(V[:, 0]).max())
+    comp, fib, tries, amb = w.find_spins(pos, xmax, 1.0, 1.0, plus_x=True)
+    assert amb == 0
+    ang = 2 * np.pi * np.arange(32) / 32
+    polys = []
+    for f, ((cx, cy), R) in enumerate(zip(ax.centers, ax.radii)):
+        for ix in (-1, 0, 1):
+            for iy in (-1, 0, 1):
+                x0, y0 = (cx - s + ix * ax.voxel) / s, (cy - s + iy * ax.voxel) / s
+                polys.append((f, np.stack([x0 + R / s * np.cos(ang), y0 + R / s * np.sin(ang)], 1)))
+    checked = 0
+    for i, (x, y, _) in enumerate(pos):
+        want_f, near = 65535, False
+        for f, P in polys:
+            if np.abs(P[:, 0].mean() - x) > 0.5 or np.abs(P[:, 1].mean() - y) > 0.5:
+                continue
+            # distance to the polygon's edges
+            a, b = P, np.roll(P, -1, 0)
+            ab = b - a
+            t = np.clip(((x - a[:, 0]) * ab[:, 0] + (y - a[:, 1]) * ab[:, 1]) / (ab ** 2).sum(1), 0, 1)
+            d = np.hypot(a[:, 0] + t * ab[:, 0] - x, a[:, 1] + t * ab[:, 1] - y).min()
+            near |= d < 1e-3
+            if _poly_contains(P, x, y):
+                want_f = f
+        if near:
+            continue
+        assert fib[i] == want_f, (i, x, y)
+        assert comp[i] == (1 if want_f != 65535 else 0)
+        checked += 1
+    assert checked > 0.95 * n
+    frac = np.mean(comp == 1)
+    sigma = np.sqrt(ax.intra_fraction * (1 - ax.intra_fraction) / n)
+    assert abs(frac - ax.intra_fraction) < 3 * sigma + 0.01, (frac, ax.intra_fraction)
