@@ -89,6 +89,7 @@ struct LabelParams {
     const uint32_t *cell_start, *cell_tri;
     uint32_t dims[3];
     float lo[3], hi[3], cell[3], inv_cell[3];
+    uint32_t exact_grid;  // 1: exact-membership grid (outside list: boxes reaching past the face)
     uint32_t *outside, *n_outside;
     uint2 *retry;
     uint32_t *n_retry;

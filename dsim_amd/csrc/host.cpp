@@ -325,37 +325,23 @@ int dsim_host::check_mesh(const dsim_mesh *mesh) {
 }
 
 // The axis of the physical mode's labelling rays.  A ray that runs along an
-// open tube never crosses its wall, so the ray must not run along the mesh's
-// long walls: per axis, sum the extent along that axis of the triangles
-// lying parallel to it (normal within 1e-3 of perpendicular).  +x is kept
-// while that sum is at most 5% of the triangles' total extent (z-aligned
-// prisms, capped or not); otherwise the axis of the smaller sum of y and z.
+// open tube never crosses its wall, and one with a component along a cut
+// tube can leave through its open end, so the ray must be perpendicular to
+// the mesh's long walls: per axis, sum the triangles' extents along it, and
+// take the smallest.  +x is kept unless its sum exceeds the smaller of the
+// other two by more than 5% (z-aligned prisms, capped or not, keep +x).
 uint32_t dsim_host::label_axis_of(const dsim_mesh *mesh) {
     const float *V = mesh->vertices;
     const uint32_t *T = mesh->triangles;
-    double score[3] = {0, 0, 0}, total = 0;
-    for (uint32_t t = 0; t < mesh->n_triangles; ++t) {
-        double a[3], b[3], c[3];
+    double score[3] = {0, 0, 0};
+    for (uint32_t t = 0; t < mesh->n_triangles; ++t)
         for (int k = 0; k < 3; ++k) {
-            a[k] = V[T[t * 3] * 3 + k];
-            b[k] = V[T[t * 3 + 1] * 3 + k];
-            c[k] = V[T[t * 3 + 2] * 3 + k];
+            const double a = V[T[t * 3] * 3 + k], b = V[T[t * 3 + 1] * 3 + k],
+                         c = V[T[t * 3 + 2] * 3 + k];
+            score[k] += std::max({a, b, c}) - std::min({a, b, c});
         }
-        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
-                             u[0] * v[1] - u[1] * v[0]};
-        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        double ext[3], emax = 0;
-        for (int k = 0; k < 3; ++k) {
-            ext[k] = std::max({a[k], b[k], c[k]}) - std::min({a[k], b[k], c[k]});
-            emax = std::max(emax, ext[k]);
-        }
-        total += emax;
-        for (int k = 0; k < 3; ++k)
-            if (std::fabs(n[k]) < 1e-3 * nn) score[k] += ext[k];
-    }
-    if (score[0] <= 0.05 * total) return 0;
+    const double other = std::min(score[1], score[2]);
+    if (score[0] <= 1.05 * other) return 0;
     return score[2] < score[1] ? 2 : 1;
 }
 
@@ -637,6 +623,42 @@ extern "C" int dsim_substrate_get_helpers(const dsim_substrate *s, float *helper
         return DSIM_OK;
     }
     if (!s->helpers.empty()) memcpy(helpers, s->helpers.data(), s->helpers.size() * 4);
+    return DSIM_OK;
+}
+
+extern "C" int dsim_alloc_array(void **dev_ptr, size_t bytes) {
+    if (!dev_ptr) return fail(DSIM_ERR_ARG, "NULL pointer");
+    *dev_ptr = nullptr;
+    if (bytes == 0) return DSIM_OK;
+    hipError_t e = hipMalloc(dev_ptr, bytes);
+    if (e != hipSuccess) {
+        *dev_ptr = nullptr;
+        return fail(DSIM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    return DSIM_OK;
+}
+
+extern "C" int dsim_free_array(void *dev_ptr) {
+    if (dev_ptr) HIP_TRY(hipFree(dev_ptr));
+    return DSIM_OK;
+}
+
+extern "C" int dsim_thread_sync(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return DSIM_OK;
+}
+
+extern "C" int dsim_copy_from_device(void *host, const void *device, size_t bytes) {
+    if (bytes == 0) return DSIM_OK;
+    if (!host || !device) return fail(DSIM_ERR_ARG, "NULL pointer");
+    HIP_TRY(hipMemcpy(host, device, bytes, hipMemcpyDeviceToHost));
+    return DSIM_OK;
+}
+
+extern "C" int dsim_copy_to_device(void *device, const void *host, size_t offset, size_t bytes) {
+    if (bytes == 0) return DSIM_OK;
+    if (!host || !device) return fail(DSIM_ERR_ARG, "NULL pointer");
+    HIP_TRY(hipMemcpy(static_cast<char *>(device) + offset, host, bytes, hipMemcpyHostToDevice));
     return DSIM_OK;
 }
 

@@ -1648,18 +1648,22 @@ hipError_t launch_label(const LabelParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Does a triangle's box start past the grid's face on the ray's side?  No
-// grid row lists such a triangle in a cell that a ray from inside the grid
-// box can reach.
-__device__ __forceinline__ bool past_grid(const LabelParams &P, float4 lo) {
+// Is the triangle (box lo, hi) taken by the outside list rather than by the
+// row walk?  AABB grids: its box starts past the grid's face on the ray's
+// side (no grid row lists it in a cell the ray can reach).  Exact-membership
+// grids: its box reaches past that face (the ray may cross it beyond the grid,
+// where no row cell lists it).
+__device__ __forceinline__ bool past_grid(const LabelParams &P, float4 lo, float4 hi) {
     const int a = (int)P.ray_axis - 1;
+    if (P.exact_grid) return (a == 0 ? hi.x : a == 1 ? hi.y : hi.z) > P.hi[a];
     return (a == 0 ? lo.x : a == 1 ? lo.y : lo.z) > P.hi[a];
 }
 
 __global__ __launch_bounds__(256) void k_label_outside(LabelParams P) {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= P.n_tri) return;
-    if (past_grid(P, P.tri_aabb[2 * t])) P.outside[atomicAdd(P.n_outside, 1u)] = t;
+    if (past_grid(P, P.tri_aabb[2 * t], P.tri_aabb[2 * t + 1]))
+        P.outside[atomicAdd(P.n_outside, 1u)] = t;
 }
 
 // Cell of coordinate v on axis a whose membership box, lo + c cell to
@@ -1696,10 +1700,11 @@ constexpr uint32_t kRowLanes = 8;
 // lies in the grid box, so the cell of the row holding it meets the
 // triangle's box and lists it (the grid lists a triangle in every cell whose
 // closed membership box meets its bounding box, and the cells' boxes tile the
-// grid box).  An exact-membership grid lists a crossed triangle in the cell
-// of the row that holds the crossing, and in a contiguous run of the row's
-// cells (a convex triangle meets a convex row of boxes in an interval), so
-// every crossing is counted once.  The kRowLanes lanes of a spin take every
+// grid box).  An exact-membership grid lists a triangle that lies inside the
+// grid box on the ray's side in the cell of the row that holds a crossing,
+// and in a contiguous run of the row's cells (a convex triangle meets a
+// convex row of boxes in an interval), so every crossing is counted once; a
+// triangle reaching past that face is taken by the outside list.  The kRowLanes lanes of a spin take every
 // kRowLanes-th cell and outside triangle and combine their sums; the sums do
 // not depend on the order, so the result equals k_label's first try.  Spins
 // with an ambiguous ray, outside the grid box, or beyond the ray's end are
@@ -1731,7 +1736,7 @@ __global__ __launch_bounds__(256) void k_label_row(LabelParams P) {
             for (uint32_t k = P.cell_start[c]; k < P.cell_start[c + 1]; ++k) {
                 const uint32_t t = P.cell_tri[k];
                 const float4 lo = P.tri_aabb[2 * t], hi = P.tri_aabb[2 * t + 1];
-                if (past_grid(P, lo)) continue;                  // in the outside list
+                if (past_grid(P, lo, hi)) continue;              // in the outside list
                 if (cA > cA0 && listed(P, c - sA, t)) continue;  // considered one cell back
                 ray_add(P, t, lo, hi, sx, sy, sz, ex, ey, ez, R);
             }

@@ -121,10 +121,17 @@ int start_request(dsim_server *S, Req *r) {
     p.spin_radius = 0.005f;
     p.sort_interval = 0;
     p.steps_per_launch = 0;
+    bool voxel = false;
+    for (int k = 0; k < 3; ++k) {
+        p.voxel_lo[k] = r->r.voxel_lo[k];
+        p.voxel_hi[k] = r->r.voxel_hi[k];
+        voxel |= p.voxel_lo[k] != 0.0f || p.voxel_hi[k] != 0.0f;
+    }
     int rc;
     if ((rc = dsim_sim_create(r->sub, &p, r->r.n_spins, &r->sim))) return rc;
     uint32_t amb = 0;
-    if ((rc = dsim_init_spins_uniform(r->sim, r->r.start_box)) ||
+    if ((rc = voxel ? dsim_init_spins_box(r->sim, p.voxel_lo, p.voxel_hi)
+                    : dsim_init_spins_uniform(r->sim, r->r.start_box)) ||
         (rc = dsim_label_spins(r->sim, r->r.label_extent[0], r->r.label_extent[1],
                                r->r.label_extent[2], 20, &amb)) ||
         (rc = dsim_moment_reset(r->sim)) || (rc = dsim_sync(r->sim)))

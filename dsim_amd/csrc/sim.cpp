@@ -642,6 +642,7 @@ extern "C" int dsim_label_spins(dsim_sim *s, float xmax, float ymax, float zmax,
         }
         L.cell_start = sub->d_cell_start.p;
         L.cell_tri = sub->d_cell_tri.p;
+        L.exact_grid = sub->kind == DSIM_GRID_TRIBOX ? 1u : 0u;
         for (int k = 0; k < 3; ++k) {
             L.dims[k] = sub->dims[k];
             L.lo[k] = sub->lo[k];

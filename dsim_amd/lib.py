@@ -20,7 +20,7 @@ ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "OVERFLOW", -6: "IO",
        -8: "NODEV"}
 NO_TRIANGLE = 0xFFFFFFFF
 NO_FIBER = 65535
-GRID_REFERENCE, GRID_AABB = 0, 1
+GRID_REFERENCE, GRID_AABB, GRID_TRIBOX = 0, 1, 2
 MODE_REFERENCE, MODE_PHYSICAL = 0, 1
 BOUNDARY_CLAMP, BOUNDARY_PERIODIC = 0, 1
 RNG_MWC, RNG_PHILOX = 1, 2
@@ -91,6 +91,11 @@ def lib():
         L.dsim_abi_version.restype = C.c_int
         sigs = {
             "dsim_device_count": [C.POINTER(C.c_int)],
+            "dsim_alloc_array": [C.POINTER(_vp), C.c_size_t],
+            "dsim_free_array": [_vp],
+            "dsim_thread_sync": [],
+            "dsim_copy_from_device": [_vp, _vp, C.c_size_t],
+            "dsim_copy_to_device": [_vp, _vp, C.c_size_t, C.c_size_t],
             "dsim_substrate_create": [C.POINTER(Mesh), C.POINTER(GridParams), C.POINTER(_vp)],
             "dsim_substrate_create_device": [C.POINTER(Mesh), C.POINTER(GridParams),
                                              C.POINTER(_vp)],
@@ -99,6 +104,7 @@ def lib():
             "dsim_substrate_get_grid": [_vp, _up, _up],
             "dsim_substrate_get_helpers": [_vp, _fp],
             "dsim_substrate_bytes": [_vp, C.POINTER(C.c_uint64)],
+            "dsim_substrate_label_axis": [_vp, _up],
             "dsim_sim_create": [_vp, C.POINTER(Params), C.c_uint32, C.POINTER(_vp)],
             "dsim_sim_destroy": [_vp],
             "dsim_set_state_aos": [_vp, _fp, _vp, _up],
@@ -144,6 +150,35 @@ def _p(a, t):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
 
+class DeviceArray:
+    """A device copy of a numpy array through the reference-style support
+    entries (dsim_alloc_array, dsim_copy_to_device, dsim_copy_from_device)."""
+
+    def __init__(self, a=None, *, nbytes=None):
+        self.nbytes = int(a.nbytes if a is not None else nbytes)
+        self.ptr = _vp()
+        check(lib().dsim_alloc_array(C.byref(self.ptr), self.nbytes))
+        if a is not None:
+            a = np.ascontiguousarray(a)
+            check(lib().dsim_copy_to_device(self.ptr, a.ctypes.data_as(_vp), 0, self.nbytes))
+
+    def get(self, dtype, shape=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        check(lib().dsim_copy_from_device(out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out if shape is None else out.reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            check(lib().dsim_free_array(self.ptr))
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Substrate:
     """Triangle mesh + uniform grid resident on the device.  device=True builds
     the records, grid and clearance on the GPU (dsim_substrate_create_device);
@@ -160,8 +195,8 @@ class Substrate:
         if grid == "reference":
             g.kind = GRID_REFERENCE
             g.dims = (C.c_uint32 * 3)(n_cubes, n_cubes, n_cubes)
-        else:
-            g.kind = GRID_AABB
+        else:  # "aabb" or "tribox" (exact triangle/box membership)
+            g.kind = GRID_TRIBOX if grid == "tribox" else GRID_AABB
             g.dims = (C.c_uint32 * 3)(*[int(d) for d in dims])
             g.lo = (C.c_float * 3)(*[float(x) for x in lo])
             g.cell = (C.c_float * 3)(*[float(x) for x in cell])
@@ -205,6 +240,12 @@ class Substrate:
         L.dsim_substrate_get_clearance.restype = C.c_int
         check(L.dsim_substrate_get_clearance(self.h, _p(out, C.c_float)))
         return out
+
+    def label_axis(self):
+        """Axis of the physical mode's labelling rays: 0 x, 1 y, 2 z."""
+        a = C.c_uint32()
+        check(lib().dsim_substrate_label_axis(self.h, C.byref(a)))
+        return a.value
 
     def nbytes(self):
         b = C.c_uint64()

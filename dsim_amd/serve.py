@@ -20,7 +20,8 @@ class Request(C.Structure):
                 ("label_extent", C.c_float * 3), ("n_lobes", C.c_uint32),
                 ("lobe_weight", C.POINTER(C.c_float)), ("lobe_steps", C.POINTER(C.c_uint32)),
                 ("n_lines", C.c_uint32), ("grads", C.POINTER(C.c_float)),
-                ("phase_constant", C.c_float)]
+                ("phase_constant", C.c_float), ("voxel_lo", C.c_float * 3),
+                ("voxel_hi", C.c_float * 3)]
 
 
 class RequestStats(C.Structure):
@@ -95,7 +96,7 @@ class Server:
 
     def submit(self, substrate, n_spins, *, seed, dt, std_dev, t2, permeability, lobes, grads,
                phase_constant, boundary=D.BOUNDARY_PERIODIC, start_box=1.0,
-               label_extent=(1.1, 1.1, 1.1)):
+               label_extent=(1.1, 1.1, 1.1), voxel=None):
         q = Request()
         q.substrate, q.n_spins, q.seed = substrate, n_spins, seed
         q.dt = dt
@@ -112,6 +113,9 @@ class Server:
         q.n_lines = len(g)
         q.grads = g.ctypes.data_as(C.POINTER(C.c_float))
         q.phase_constant = phase_constant
+        if voxel is not None:  # ((lo x, y, z), (hi x, y, z)): non-cubic periodic voxel
+            q.voxel_lo[:] = [float(v) for v in voxel[0]]
+            q.voxel_hi[:] = [float(v) for v in voxel[1]]
         t = C.c_uint64()
         _check(_lib().dsim_server_submit(self.h, C.byref(q), C.byref(t)))
         self._lines[t.value] = len(g)

@@ -115,6 +115,20 @@ int dsim_abi_version(void);
 /* Number of visible HIP devices (0 on a host without a GPU). */
 int dsim_device_count(int *n);
 
+/* Device-memory support entries of the reference's C ABI
+ * (dSim3/src/spinSystem.cuh:6-13, defined in spinSystem.cu:35-88), for
+ * callers that keep their own device arrays (as SpinSystem does): */
+/* allocateArray (spinSystem.cu:35-38): device allocation of `bytes`. */
+int dsim_alloc_array(void **dev_ptr, size_t bytes);
+/* freeArray (spinSystem.cu:46-49). */
+int dsim_free_array(void *dev_ptr);
+/* threadSync (spinSystem.cu:57-60): waits for all work on the device. */
+int dsim_thread_sync(void);
+/* copyArrayFromDevice (spinSystem.cu:68-75) without the GL buffer mapping. */
+int dsim_copy_from_device(void *host, const void *device, size_t bytes);
+/* copyArrayToDevice (spinSystem.cu:83-86): device + offset <- host. */
+int dsim_copy_to_device(void *device, const void *host, size_t offset, size_t bytes);
+
 /* ---- substrate ---------------------------------------------------------- */
 /* Replaces the device half of SpinSystem::initFibers (triangle helpers and
  * their texture binds, spinSystem.cpp:612-665) and SpinSystem::populateCubes
@@ -148,10 +162,12 @@ int dsim_substrate_get_helpers(const dsim_substrate *s, float *helpers);
  * walked a path shorter than this clearance. */
 int dsim_substrate_get_clearance(const dsim_substrate *s, float *clearance);
 /* No reference counterpart: the axis of the physical mode's labelling rays
- * (0 +x, 1 +y, 2 +z), chosen from the mesh so that the rays do not run along
- * its long walls (findSpinsInFibers casts along the spin's dominant axis,
- * spinSystem.cpp:1677-1683, which misses an open tube along that axis).  +x
- * unless the triangles parallel to x hold more than 5% of the mesh's extent. */
+ * (0 +x, 1 +y, 2 +z), chosen from the mesh so that the rays run across its
+ * long walls (findSpinsInFibers casts along the spin's dominant axis,
+ * spinSystem.cpp:1677-1683, which misses an open tube along that axis and
+ * can leave a cut tube through its end): the axis along which the triangles'
+ * extents sum least, +x unless that sum exceeds the smaller of the others by
+ * more than 5%. */
 int dsim_substrate_label_axis(const dsim_substrate *s, uint32_t *axis);
 /* Device bytes held by the substrate. */
 int dsim_substrate_bytes(const dsim_substrate *s, uint64_t *bytes);

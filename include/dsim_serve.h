@@ -47,6 +47,8 @@ typedef struct {
     uint32_t n_lines;         /* readout lines */
     const float *grads;       /* [n_lines*3] gradScaled (include/dsim.h units) */
     float phase_constant;     /* 2*pi*gamma*dt */
+    float voxel_lo[3];        /* periodic voxel (dsim_params); when set, the spins start */
+    float voxel_hi[3];        /* uniform in it and start_box is not used; all zero: cubic */
 } dsim_request;
 
 typedef struct {
@@ -64,7 +66,8 @@ int dsim_server_create(uint64_t hbm_budget, uint32_t round_steps, uint32_t max_a
 /* Stops the background loop if running, waits for the device and frees every
  * request.  Substrates stay owned by the caller. */
 int dsim_server_destroy(dsim_server *srv);
-/* Registers a resident substrate (created on the device with an AABB grid). */
+/* Registers a resident substrate (created on the device with an AABB or an
+ * exact-membership grid). */
 int dsim_server_add_substrate(dsim_server *srv, dsim_substrate *sub, uint32_t *id);
 /* Queues a request (its arrays are copied).  Rejected with DSIM_ERR_ARG if it
  * alone exceeds the budget or the spin limit. */

@@ -111,6 +111,8 @@ def _declare(L):
     L.orc_populate_aabb.argtypes = [_fp, _up, C.c_uint32, _up, _fp, _fp, C.c_float, _up, _up,
                                     C.c_uint32]
     L.orc_populate_aabb.restype = C.c_uint32
+    L.orc_populate_aabb_plane.argtypes = L.orc_populate_aabb.argtypes
+    L.orc_populate_aabb_plane.restype = C.c_uint32
     L.orc_tri_coll_detect.argtypes = [P(WorldC), F3, F3, C.c_uint32]
     L.orc_tri_coll_detect.restype = Coll
     L.orc_mt_coll_detect.argtypes = [P(WorldC), F3, F3, C.c_uint32]
@@ -225,7 +227,7 @@ def populate_cubes(vertices, trgls, n_cubes=50):
     return cnt, lst[: n_cubes ** 3 * maxc].reshape(n_cubes ** 3, maxc), maxc
 
 
-def populate_aabb(vertices, trgls, dims, lo, cell, eps):
+def populate_aabb(vertices, trgls, dims, lo, cell, eps, plane=False):
     V = np.ascontiguousarray(vertices, np.float32).ravel()
     T = np.ascontiguousarray(trgls, np.uint32).ravel()
     n = T.size // 3
@@ -236,9 +238,10 @@ def populate_aabb(vertices, trgls, dims, lo, cell, eps):
     cnt = np.zeros(nc, np.uint32)
     args = [_p(V, C.c_float), _p(T, C.c_uint32), n, _p(d, C.c_uint32), _p(l, C.c_float),
             _p(c, C.c_float), C.c_float(eps)]
-    maxc = lib().orc_populate_aabb(*args, _p(cnt, C.c_uint32), None, 0)
+    fn = lib().orc_populate_aabb_plane if plane else lib().orc_populate_aabb
+    maxc = fn(*args, _p(cnt, C.c_uint32), None, 0)
     lst = np.zeros(max(1, nc * maxc), np.uint32)
-    lib().orc_populate_aabb(*args, _p(cnt, C.c_uint32), _p(lst, C.c_uint32), maxc)
+    fn(*args, _p(cnt, C.c_uint32), _p(lst, C.c_uint32), maxc)
     return cnt, lst[: nc * maxc].reshape(nc, maxc), maxc
 
 
@@ -276,14 +279,14 @@ class World:
             self.ref_cell_math = 1
             if grid_arrays is None:
                 grid_arrays = populate_cubes(self.vertices, self.trgls, n_cubes)
-        else:
+        else:  # "aabb", or "aabb_plane" (the plane must pass through the cell too)
             self.dims = tuple(int(d) for d in dims)
             self.lo = tuple(float(np.float32(x)) for x in lo)
             self.cell = tuple(float(np.float32(x)) for x in cell)
             self.ref_cell_math = 0
             if grid_arrays is None:
                 grid_arrays = populate_aabb(self.vertices, self.trgls, self.dims, self.lo,
-                                            self.cell, eps)
+                                            self.cell, eps, plane=grid == "aabb_plane")
         self.cell_count, self.cell_list, self.maxc = grid_arrays
         self.cell_count = np.ascontiguousarray(self.cell_count, np.uint32)
         self.cell_list = np.ascontiguousarray(self.cell_list, np.uint32)

@@ -302,10 +302,32 @@ uint32_t orc_populate_cubes(const float *V, const uint32_t *T, uint32_t n_tri, u
 }
 
 /* Build-defined conservative membership: triangle AABB grown by eps overlaps
- * the cell box [lo+i*cell, lo+(i+1)*cell] (inclusive). */
+ * the cell box [lo+i*cell, lo+(i+1)*cell] (inclusive); with `plane` set the
+ * triangle's plane must also pass within eps of the box (a triangle lies in
+ * its plane, so this drops only cells the triangle cannot touch). */
+static uint32_t populate_boxes(const float *V, const uint32_t *T, uint32_t n_tri,
+                               const uint32_t dims[3], const float lo[3], const float cell[3],
+                               float eps, uint32_t *count, uint32_t *list, uint32_t max_per_cell,
+                               int plane);
+
 uint32_t orc_populate_aabb(const float *V, const uint32_t *T, uint32_t n_tri, const uint32_t dims[3],
                            const float lo[3], const float cell[3], float eps, uint32_t *count,
                            uint32_t *list, uint32_t max_per_cell)
+{
+    return populate_boxes(V, T, n_tri, dims, lo, cell, eps, count, list, max_per_cell, 0);
+}
+
+uint32_t orc_populate_aabb_plane(const float *V, const uint32_t *T, uint32_t n_tri,
+                                 const uint32_t dims[3], const float lo[3], const float cell[3],
+                                 float eps, uint32_t *count, uint32_t *list, uint32_t max_per_cell)
+{
+    return populate_boxes(V, T, n_tri, dims, lo, cell, eps, count, list, max_per_cell, 1);
+}
+
+static uint32_t populate_boxes(const float *V, const uint32_t *T, uint32_t n_tri,
+                               const uint32_t dims[3], const float lo[3], const float cell[3],
+                               float eps, uint32_t *count, uint32_t *list, uint32_t max_per_cell,
+                               int plane)
 {
     size_t ncells = (size_t)dims[0] * dims[1] * dims[2];
     memset(count, 0, ncells * sizeof(uint32_t));
@@ -337,6 +359,24 @@ uint32_t orc_populate_aabb(const float *V, const uint32_t *T, uint32_t n_tri, co
                 for (int64_t x = i0[0]; x <= i1[0]; ++x) {
                     float bx0 = lo[0] + (float)x * cell[0], bx1 = lo[0] + (float)(x + 1) * cell[0];
                     if (bx0 > mx[0] || mn[0] > bx1) continue;
+                    if (plane) {
+                        const double b0[3] = {bx0, by0, bz0}, b1[3] = {bx1, by1, bz1};
+                        double A[3], B[3], Cv[3], nrm[3], dist = 0.0, rad = 0.0, nn = 0.0;
+                        for (int k = 0; k < 3; ++k) {
+                            A[k] = V[T[t * 3] * 3 + k];
+                            B[k] = V[T[t * 3 + 1] * 3 + k] - A[k];
+                            Cv[k] = V[T[t * 3 + 2] * 3 + k] - A[k];
+                        }
+                        nrm[0] = B[1] * Cv[2] - B[2] * Cv[1];
+                        nrm[1] = B[2] * Cv[0] - B[0] * Cv[2];
+                        nrm[2] = B[0] * Cv[1] - B[1] * Cv[0];
+                        for (int k = 0; k < 3; ++k) {
+                            dist += nrm[k] * (0.5 * (b0[k] + b1[k]) - A[k]);
+                            rad += 0.5 * (b1[k] - b0[k]) * fabs(nrm[k]);
+                            nn += nrm[k] * nrm[k];
+                        }
+                        if (fabs(dist) > rad + (double)eps * sqrt(nn)) continue;
+                    }
                     size_t idx = ((size_t)z * dims[1] + (size_t)y) * dims[0] + (size_t)x;
                     if (list) list[idx * max_per_cell + count[idx]] = t;
                     count[idx] += 1;

@@ -141,6 +141,13 @@ uint32_t orc_populate_aabb(const float *vertices, const uint32_t *trgls, uint32_
                            float eps, uint32_t *count, uint32_t *list,
                            uint32_t max_per_cell);
 
+/* The same with the triangle's plane required to pass within eps of the cell
+ * (for long triangles that do not run along an axis). */
+uint32_t orc_populate_aabb_plane(const float *vertices, const uint32_t *trgls, uint32_t n_tri,
+                                 const uint32_t dims[3], const float lo[3], const float cell[3],
+                                 float eps, uint32_t *count, uint32_t *list,
+                                 uint32_t max_per_cell);
+
 /* ---- collision ------------------------------------------------------------ */
 orc_coll orc_tri_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri);
 orc_coll orc_mt_coll_detect(const orc_world *w, orc_f3 o, orc_f3 p, uint32_t tri);
