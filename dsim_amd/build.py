@@ -32,7 +32,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFOR
              "-I" + os.path.join(ROCM, "include"), "-I" + os.path.join(ROOT, "include"),
              "-Wall", "-Wno-misleading-indentation", "-Wno-unused-result", "-pthread"]
 
-HIP_SOURCES = ["kernels.hip", "grid.hip"]
+HIP_SOURCES = ["kernels.hip", "grid.hip", "nalloc.hip"]
 CXX_SOURCES = ["host.cpp", "sim.cpp", "formats.cpp", "serve.cpp", "phase.cpp"]
 MAIN_SOURCES = ["spin_system.cpp", "dsim_main.cpp"]
 

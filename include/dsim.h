@@ -242,6 +242,26 @@ int dsim_signal_moment(dsim_sim *sim, uint32_t ndir, const float *grads_scaled,
 /* Host copy of the moment accumulators [n*3] and x0 [n*3] (either may be NULL). */
 int dsim_get_moment(dsim_sim *sim, float *moment, float *x0);
 
+/* ---- allocation-free readout ------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): dsim_signal_moment over device arrays the caller owns, with its
+ * partial sums in a fixed device array of the library (7.9 MB per GPU, kept
+ * until the process ends) instead of allocated memory.  Inputs are device
+ * pointers in the handle's field-major layout: mag [n], moment [3][n], x0
+ * [3][n] (may be NULL when wsum is 0), label [n] (compartment | fibre << 8),
+ * grads_scaled [ndir*3]; out is a device array [ndir*5] laid out as
+ * dsim_signal_moment's.  Work is queued on `stream` (a hipStream_t, NULL for
+ * the null stream) and not waited for.  At most DSIM_NALLOC_MAX_DIRECTIONS
+ * directions per call (DSIM_ERR_ARG beyond).  Every call on one GPU shares
+ * the array: calls that overlap on one GPU race and must be serialised by
+ * the caller (one stream, or events). */
+#define DSIM_NALLOC_MAX_DIRECTIONS 256
+int dsim_signal_moment_nalloc(uint32_t n, const float *mag, const float *moment, const float *x0,
+                              float wsum, const uint32_t *label, uint32_t ndir,
+                              const float *grads_scaled, float phase_constant, double *out,
+                              void *stream);
+
 /* ---- timing ------------------------------------------------------------- */
 int dsim_sync(dsim_sim *sim);
 /* Milliseconds spent in step-kernel launches since the last reset, measured
