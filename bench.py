@@ -9,8 +9,16 @@ ms/um^2 shells plus b=0, fp32.  One bench step is one complete acquisition:
 every spin walks 12,000 steps once (the gradient-moment accumulation serves
 all 65 gradient lines) and the 65 signals are read out in FP64.
 
+Beside the headline line the JSON carries: the batched configuration (256
+crossing-bundle substrates advanced together by the server's batched
+launches), the served configuration (capacity, latency against solo runs,
+and the 50 req/s stream over a fixed window), the waveform-phase path (4e6
+spins x 1024 waveforms on the matrix cores), the reference-semantics path,
+and CPU baselines (the oracle on this host's cores).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config scale|small]
                        [--cell-um H] [--sort S] [--no-autotune] [--no-cpu-baseline]
+                       [--batched-substrates N] [--served-requests N] [--waveform-spins N]
 """
 from __future__ import annotations
 
@@ -52,6 +60,36 @@ def flops_per_spin_step(n_tri, n_coll, n_ch=3):
     9, wrap 6, T2 2, moment channels 7 each, Moller-Trumbore 45 per candidate
     whose box overlaps the step's box, reflection 21 per membrane hit."""
     return 50 + 7 * n_ch + 45 * n_tri + 21 * n_coll
+
+
+def host_cpu():
+    """The host's CPU as this process sees it: the model (/proc/cpuinfo),
+    logical CPUs (nproc), the CPUs this process may run on (affinity), the
+    cgroup's CPU quota, and the threads the CPU baselines use: every CPU the
+    process may run on, within the quota."""
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"model": model, "nproc": nproc, "affinity_cpus": aff, "cgroup_cpu_quota": quota,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "threads_used": threads}
 
 
 def cpu_baseline(env, target_s, threads):
@@ -98,59 +136,153 @@ def cpu_baseline(env, target_s, threads):
                 n_tri=n_cand / max(1, n_steps), n_coll=n_hit / max(1, n_steps))
 
 
-def served_bench(n_requests, plain_requests, round_steps=256):
-    """The served form against the plain one on a stream of requests over
-    resident substrates: capacity (every request queued at once) in
-    simulations per hour, the same requests one at a time through the plain
-    handle, and Poisson streams at 40 % and 70 % of the measured capacity for
-    the latency percentiles (each request's latency also against its own
-    solo run)."""
-    from dsim_amd import served as SVD
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
-    sc = SVD.StreamConfig(n_requests=n_requests)
+
+def batched_bench(n_substrates, plain_sample, round_steps=256):
+    """The batched configuration: crossing-bundle substrates (axon counts
+    50-5000, mean radii 0.5-3 um, two bundles at 0-90 degrees snapped to a
+    lattice direction), each with one request of log-uniform 1e4-1e6 spins
+    and 6-256 PGSE directions at one b <= 3 ms/um^2 (delta/Delta/TE
+    10/40/60 ms), all advanced together by the batched step launches of the
+    server; cell sizes from the autotuner.  The plain path runs a random
+    sample of the same requests one at a time for the comparison (spin-steps/s
+    on both sides: the requests differ in size)."""
+    from dsim_amd import batched as B
+
+    cs = B.CrossingSet(n_substrates=n_substrates)
     t0 = time.perf_counter()
-    envs = SVD.make_substrates(sc, log)
-    log(f"served: {sc.n_substrates} resident substrates in {time.perf_counter() - t0:.1f} s")
-    reqs = SVD.make_requests(sc, envs)
-    out, mk = SVD.run_served(envs, reqs, realtime=False, round_steps=round_steps)
-    cap = len(out) / mk
-    log(f"served: {len(out)} requests ({SVD.work(reqs):.3e} spin-steps) in {mk:.2f} s")
-    sub = reqs[:plain_requests]
-    plain = SVD.run_plain(envs, sub)
+    items = list(B.meshes(cs))
+    t_mesh = time.perf_counter() - t0
+    ntri = sum(len(it[3]) for it in items)
+    log(f"batched: {len(items)} crossing meshes ({ntri} triangles) in {t_mesh:.1f} s")
+    rs = sorted(items, key=lambda it: float(np.mean(np.concatenate([it[1].a_r, it[1].b_r]))))
+    t0 = time.perf_counter()
+    ratio, score = B.tune_cell_ratio(cs, [rs[0], rs[len(rs) // 2], rs[-1]], log=log)
+    t_tune = time.perf_counter() - t0
+    log(f"batched: cell = {ratio} x mean radius ({t_tune:.1f} s)")
+    t0 = time.perf_counter()
+    envs = []
+    for k, it in enumerate(items):
+        envs.append(B.make_env(cs, it, ratio))
+        items[k] = None  # the host arrays are not needed any more
+    t_build = time.perf_counter() - t0
+    sub_bytes = sum(e["bytes"] for e in envs)
+    log(f"batched: {len(envs)} device substrates in {t_build:.1f} s ({sub_bytes / 1e9:.1f} GB)")
+    reqs = B.batched_requests(cs, envs)
+    wk = B.work(reqs)
+    t0 = time.perf_counter()
+    out, span = B.run_server(cs, envs, reqs, round_steps=round_steps)
+    wall = time.perf_counter() - t0
+    rate = wk / span
+    log(f"batched: {len(out)} requests, {sum(q['n'] for q in reqs)} spins, {wk:.3e} spin-steps in "
+        f"{span:.2f} s: {rate:.3e} spin-steps/s, {len(out) / span * 3600:.0f} simulations/hour")
+    idx = np.sort(np.random.default_rng(3).choice(len(reqs), min(plain_sample, len(reqs)),
+                                                  replace=False))
+    sample = [reqs[i] for i in idx]
+    plain = B.run_plain(cs, envs, sample)
     t_plain = sum(t for t, _ in plain)
-    same = all(np.array_equal(o["signals"], sig) for o, (_, sig) in zip(out, plain))
-    log(f"plain: {len(sub)} requests one at a time in {t_plain:.2f} s; served signals "
-        f"{'bit-identical' if same else 'DIFFER'}")
+    same = all(np.array_equal(out[i]["signals"], sig) for i, (_, sig) in zip(idx, plain))
+    plain_rate = B.work(sample) / t_plain
+    log(f"batched: plain path on {len(sample)} of the requests {plain_rate:.3e} spin-steps/s; "
+        f"batched signals {'bit-identical' if same else 'DIFFER'}")
+    res = {"workload": (f"{len(envs)} crossing-bundle substrates ({cs.axons[0]}-{cs.axons[1]} axons, "
+                        f"mean radius {cs.radius[0]}-{cs.radius[1]} um, 32-gon, crossing "
+                        f"0-90 deg on lattice directions), one request each: 1e4-1e6 spins "
+                        f"(log-uniform), 6-256 directions at b 0.5-3 ms/um^2 + b=0, PGSE "
+                        f"10/40/60 ms"),
+           "substrates": len(envs), "requests": len(reqs), "spins": int(sum(q["n"] for q in reqs)),
+           "spin_steps": wk, "seconds": span, "spin_steps_per_s": rate,
+           "sims_per_hour": len(out) / span * 3600, "plain_sample": len(sample),
+           "plain_spin_steps_per_s": plain_rate, "speedup_vs_plain": rate / plain_rate,
+           "bit_identical_to_plain": bool(same), "round_steps": round_steps,
+           "cell_ratio": ratio, "cell_ratio_candidates": score,
+           "cells_um": [round(min(e["cell_um"] for e in envs), 3),
+                        round(max(e["cell_um"] for e in envs), 3)],
+           "triangles": ntri, "substrate_bytes": sub_bytes,
+           "setup_s": {"meshes": round(t_mesh, 1), "tuning": round(t_tune, 1),
+                       "device_builds": round(t_build, 1)}}
+    return res, cs, envs
+
+
+def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_rate=50.0,
+                 overload_window=8.0, overload_drain=20.0, overload_active_spins=4 << 20):
+    """The served configuration on resident crossing substrates.  A stream
+    scaled to one GPU (requests of 1e4-2e5 spins, protocols as configured):
+    capacity with every request queued at once, the same requests one at a
+    time through the plain handle, and Poisson streams at a fraction of the
+    capacity for the latency percentiles, each request's latency also against
+    its own solo (plain) time.  Then the configured stream (Poisson 50 req/s,
+    1e4-2e6 spins) over a fixed arrival window: the requests done within the
+    window plus a drain period, their latencies, and the sustained rate."""
+    from dsim_amd import batched as B
+
+    reqs = B.stream_requests(cs, envs, n_requests, spins=(1e4, 2e5))
+    out, span = B.run_server(cs, envs, reqs, round_steps=round_steps)
+    cap = len(out) / span
+    log(f"served: {len(out)} requests ({B.work(reqs):.3e} spin-steps) in {span:.2f} s: "
+        f"{cap * 3600:.0f} simulations/hour")
+    plain = B.run_plain(cs, envs, reqs)
     solo = np.array([t for t, _ in plain])
+    same = all(np.array_equal(o["signals"], sig) for o, (_, sig) in zip(out, plain))
+    log(f"plain: the same {len(reqs)} requests one at a time in {solo.sum():.2f} s "
+        f"({len(reqs) / solo.sum() * 3600:.0f} simulations/hour); served signals "
+        f"{'bit-identical' if same else 'DIFFER'}")
+    by_seed = {q["seed"]: t for q, t in zip(reqs, solo)}
     streams = []
-    for load in (0.4, 0.7):
+    for load in loads:
         rate = load * cap
-        live, _ = SVD.run_served(envs, SVD.make_requests(sc, envs, rate=rate), realtime=True,
-                                 round_steps=round_steps)
+        live, _ = B.run_server(cs, envs, B.stream_requests(cs, envs, n_requests, rate=rate,
+                                                           spins=(1e4, 2e5)),
+                               realtime=True, round_steps=round_steps)
         lat = np.array([o["latency"] for o in live])
-        vs = lat[:len(solo)] / solo
-        streams.append({"load": load, "rate_per_s": rate,
-                        "p50_latency_s": float(np.percentile(lat, 50)),
-                        "p99_latency_s": float(np.percentile(lat, 99)),
-                        "p50_latency_vs_solo": float(np.percentile(vs, 50)),
-                        "p99_latency_vs_solo": float(np.percentile(vs, 99))})
-        log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity): latency p50 "
-            f"{np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s; vs solo p50 "
-            f"{np.percentile(vs, 50):.2f}x, p99 {np.percentile(vs, 99):.2f}x")
-    for e in envs:
-        e["sub"].close()
-    return {"requests": len(reqs), "substrates": sc.n_substrates, "round_steps": round_steps,
-            "sims_per_hour": cap * 3600, "plain_sims_per_hour": len(sub) / t_plain * 3600,
-            "speedup_vs_plain": cap / (len(sub) / t_plain),
-            "served_spin_steps_per_s": SVD.work(reqs) / mk,
-            "streams": streams,
-            "bit_identical_to_plain": bool(same),
-            "workload": (f"{sc.n_substrates} substrates of {sc.axons[0]}-{sc.axons[1]} axons "
-                         f"(mean radius {sc.radius[0]}-{sc.radius[1]} um, {sc.ngon}-gon), "
-                         f"requests of {sc.spins[0]:.0e}-{sc.spins[1]:.0e} spins (log-uniform), "
-                         f"{sc.lines[0]}-{sc.lines[1]} lines, b {sc.b[0]}-{sc.b[1]} ms/um^2, "
-                         f"delta {sc.delta[0]}-{sc.delta[1]} ms, Delta {sc.Delta[0]}-{sc.Delta[1]} ms; "
-                         f"plain = first {len(sub)} requests one at a time")}
+        vs = np.array([o["latency"] / by_seed[o["req"]["seed"]] for o in live])
+        streams.append({"load": load, "rate_per_s": rate, "requests": len(live),
+                        "p50_latency_s": _pct(lat, 50), "p99_latency_s": _pct(lat, 99),
+                        "p50_latency_vs_solo": _pct(vs, 50), "p99_latency_vs_solo": _pct(vs, 99)})
+        log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity), {len(live)} requests: latency "
+            f"p50 {_pct(lat, 50):.3f} s, p99 {_pct(lat, 99):.3f} s; vs solo p50 "
+            f"{_pct(vs, 50):.2f}x, p99 {_pct(vs, 99):.2f}x")
+    # the configured stream: far beyond one GPU's capacity, so a finite window
+    n_over = int(overload_rate * overload_window * 1.5) + 10
+    over = B.stream_requests(cs, envs, n_over, rate=overload_rate, seed=9)
+    sub = [q for q in over if q["arrival"] <= overload_window]
+    # admission holds the spins in flight to what fills the device (FIFO):
+    # without the limit every queued request shares the rounds and none ends
+    # within the window
+    o2, span2 = B.run_server(cs, envs, sub, realtime=True, round_steps=round_steps,
+                             window_s=overload_window + overload_drain,
+                             max_active_spins=overload_active_spins)
+    done = [o for o in o2 if o["done"] is not None]
+    lat = [o["latency"] for o in done]
+    wait = [o["queued"] for o in done]
+    svc = [o["service"] for o in done]
+    wdone = B.work([o["req"] for o in done])
+    T = overload_window + overload_drain
+    overload = {"rate_per_s": overload_rate, "arrival_window_s": overload_window,
+                "max_active_spins": overload_active_spins,
+                "measured_until_s": T, "submitted": len(sub), "completed": len(done),
+                "offered_spin_steps_per_s": B.work(sub) / overload_window,
+                "sustained_spin_steps_per_s": wdone / T,
+                "sustained_sims_per_hour": len(done) / T * 3600,
+                "completed_p50_latency_s": _pct(lat, 50), "completed_p99_latency_s": _pct(lat, 99),
+                "completed_p50_queue_s": _pct(wait, 50), "completed_p99_queue_s": _pct(wait, 99),
+                "completed_p50_service_s": _pct(svc, 50), "completed_p99_service_s": _pct(svc, 99)}
+    log(f"configured stream: {len(sub)} requests at {overload_rate:.0f}/s over {overload_window:.0f} s "
+        f"(offered {overload['offered_spin_steps_per_s']:.2e} spin-steps/s); {len(done)} done within "
+        f"{T:.0f} s (sustained {overload['sustained_spin_steps_per_s']:.2e} spin-steps/s); latency p50 "
+        f"{_pct(lat, 50)} s, p99 {_pct(lat, 99)} s")
+    return {"workload": (f"{len(envs)} resident crossing substrates (drawn as the batched set); "
+                         f"requests: 6-256 directions, b 0.5-5 ms/um^2, delta 5-20 ms, Delta "
+                         f"20-60 ms; capacity and latency streams of {n_requests} requests of "
+                         f"1e4-2e5 spins, the configured stream of 1e4-2e6 spins at "
+                         f"{overload_rate:.0f} req/s"),
+            "requests": len(reqs), "substrates": len(envs), "round_steps": round_steps,
+            "sims_per_hour": cap * 3600, "plain_sims_per_hour": len(reqs) / solo.sum() * 3600,
+            "speedup_vs_plain": cap / (len(reqs) / solo.sum()),
+            "served_spin_steps_per_s": B.work(reqs) / span, "streams": streams,
+            "bit_identical_to_plain": bool(same), "configured_stream": overload}
 
 
 def reference_mode_bench(n, steps, cpu_seconds, threads, cpu=True):
@@ -396,9 +528,14 @@ def main():
     ap.add_argument("--cell-um", type=float, default=0.0)
     ap.add_argument("--sort", type=int, default=-1)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--served-requests", type=int, default=120,
-                    help="requests in the served-form comparison (0 skips it)")
-    ap.add_argument("--plain-requests", type=int, default=40)
+    ap.add_argument("--batched-substrates", type=int, default=256,
+                    help="crossing substrates of the batched measurement (0 skips it and the "
+                         "served one)")
+    ap.add_argument("--plain-sample", type=int, default=16,
+                    help="batched requests also run one at a time through the plain handle")
+    ap.add_argument("--served-substrates", type=int, default=64)
+    ap.add_argument("--served-requests", type=int, default=200,
+                    help="requests of the served capacity and latency streams (0 skips them)")
     ap.add_argument("--round-steps", type=int, default=256,
                     help="steps per batched round of the served form")
     ap.add_argument("--waveform-spins", type=int, default=4_000_000,
@@ -493,9 +630,11 @@ def main():
 
     cpu = None
     n_tri, n_coll = None, None
+    hcpu = host_cpu()
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        threads = hcpu["threads_used"]
         cpu = cpu_baseline(env, args.cpu_seconds, threads)
+        cpu["host"] = hcpu
         n_tri, n_coll = cpu.pop("n_tri"), cpu.pop("n_coll")
         log(f"cpu baseline {cpu['value']:.3e} spin-steps/s ({cpu['sample']}); work model "
             f"n_tri {n_tri:.3f}, n_coll {n_coll:.4f}")
@@ -510,20 +649,28 @@ def main():
                 "flops_per_spin_step": F, "n_tri": n_tri, "n_coll": n_coll,
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "spin_steps_per_launch": n * steps_per_acq * args.steps / max(1, launches)}
-    served = None
-    if args.served_requests:
+    batched = served = None
+    if args.batched_substrates:
         sim.close()
         sub.close()
-        served = served_bench(args.served_requests, min(args.plain_requests, args.served_requests),
-                              args.round_steps)
+        batched, cs, envs = batched_bench(args.batched_substrates, args.plain_sample,
+                                          args.round_steps)
+        keep = envs[:args.served_substrates]
+        for e in envs[args.served_substrates:]:
+            e["sub"].close()
+        if args.served_requests:
+            served = served_bench(cs, keep, args.served_requests, args.round_steps)
+        for e in keep:
+            e["sub"].close()
     waves = None
     if args.waveform_spins:
         waves = waveform_bench(args.waveform_spins, args.waveforms, 8192, args.chunk_steps)
     refmode = None
     if args.reference_spins:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        refmode = reference_mode_bench(args.reference_spins, 1000, 5.0, threads,
+        refmode = reference_mode_bench(args.reference_spins, 1000, 5.0, hcpu["threads_used"],
                                        cpu=not args.no_cpu_baseline)
+        if refmode.get("cpu_baseline"):
+            refmode["gpu_over_cpu"] = refmode["value"] / refmode["cpu_baseline"]["value"]
     out = {
         "metric": "spin-steps/s at 1M spins",
         "value": value,
@@ -550,10 +697,12 @@ def main():
         "labelling": labelling,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
         "served": served,
         "waveforms": waves,
         "reference_mode": refmode,
-        "host": platform.processor() or platform.machine(),
+        "host": dict(hcpu, arch=platform.machine()),
+        "gpu_over_cpu": value / cpu["value"] if cpu else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -35,6 +35,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFOR
 HIP_SOURCES = ["kernels.hip", "grid.hip", "nalloc.hip"]
 CXX_SOURCES = ["host.cpp", "sim.cpp", "formats.cpp", "serve.cpp", "phase.cpp"]
 MAIN_SOURCES = ["spin_system.cpp", "dsim_main.cpp"]
+EXAMPLE_SOURCES = ["spin_system.cpp", "example_spin_system.cpp"]
+EXAMPLE = os.path.join(HERE, "bin", "dsim_example")
 
 
 def _run(cmd):
@@ -62,6 +64,9 @@ def build(verbose: bool = False) -> str:
         o = os.path.join(BUILD, s + ".o")
         jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
         main_objs.append(o)
+    ex_objs = [os.path.join(BUILD, s + ".o") for s in EXAMPLE_SOURCES]
+    jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, EXAMPLE_SOURCES[1]), "-o",
+                 ex_objs[1]])
     with ThreadPoolExecutor(max_workers=4) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
@@ -71,9 +76,10 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if not main_objs:
         return LIB
-    _run(["g++", "-o", BIN, *main_objs, "-L" + HERE, "-ldsim", "-Wl,-rpath,$ORIGIN/..",
-          "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
-          "-pthread"])
+    link = ["-L" + HERE, "-ldsim", "-Wl,-rpath,$ORIGIN/..", "-L" + os.path.join(ROCM, "lib"),
+            "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-pthread"]
+    _run(["g++", "-o", BIN, *main_objs, *link])
+    _run(["g++", "-o", EXAMPLE, *ex_objs, *link])
     return LIB
 
 

@@ -131,12 +131,16 @@ hipError_t launch_label(const LabelParams &p, hipStream_t s);
 // grid fields of p must be set; *n_outside and *n_retry must be zero.
 hipError_t launch_label_row(const LabelParams &p, hipStream_t s);
 // Batched physical-mode walk over a device-resident job table (periodic or
-// clamped boundary for every job of the launch).
+// clamped boundary for every job of the launch).  recs: where the jobs' grid
+// entries keep their records (kRecsCellMajor: every job's grid has cell-major
+// copies; kRecsIndirect: none has; kRecsEither: mixed, decided per job).
+enum { kRecsEither = 0, kRecsCellMajor = 1, kRecsIndirect = 2 };
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
-                             hipStream_t s);
+                             int recs, hipStream_t s);
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
                                uint32_t *label, uint32_t *id, uint32_t *wrap, const float lo[3],
-                               const float hi[3], uint32_t key0, uint32_t key1, hipStream_t s);
+                               const float hi[3], uint32_t key0, uint32_t key1, uint64_t draw,
+                               hipStream_t s);
 hipError_t launch_moment_reset(uint32_t n, const float *x, const float *y, const float *z,
                                float *x0, float *disp, float *mom, float *momc, float *mag,
                                hipStream_t s);

@@ -37,7 +37,6 @@ using dsim_host::fail;
 namespace {
 
 constexpr uint32_t kFar = 0xFFFFFFFFu;
-constexpr uint32_t kExactRings = 3;
 constexpr uint32_t kBlock = 256;
 
 __device__ __forceinline__ int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
@@ -49,6 +48,7 @@ struct Geom {
     float lo[3], cell[3];
     float eps;
     float L;  // reference grid: cube edge 2/n
+    int64_t exact_rings;  // cells at ring distance <= this refine the clearance
 };
 
 struct F3 {
@@ -336,7 +336,7 @@ __device__ float float_sqrt_floor(double x) {
 }
 
 // Clearance of each cell, as compute_clearance in host.cpp: from the ring
-// distance alone beyond kExactRings, else the smallest box-to-box distance to
+// distance alone beyond g.exact_rings, else the smallest box-to-box distance to
 // the triangles of the rings around the cell.
 __global__ __launch_bounds__(kBlock) void k_clearance(Geom g, const uint32_t *ring,
                                                       const uint32_t *start, const uint32_t *tri,
@@ -350,8 +350,9 @@ __global__ __launch_bounds__(kBlock) void k_clearance(Geom g, const uint32_t *ri
     double hmin = (double)g.cell[0];
     if ((double)g.cell[1] < hmin) hmin = (double)g.cell[1];
     if ((double)g.cell[2] < hmin) hmin = (double)g.cell[2];
-    if (rc > kExactRings) {
-        const double best = (double)(rc - 1) * hmin;
+    if ((int64_t)rc > g.exact_rings) {
+        // an occupied cell (ring 0) has no clearance
+        const double best = rc > 0 ? (double)(rc - 1) * hmin : 0.0;
         float f = (float)best;
         if ((double)f > best) f = nextafterf(f, 0.0f);
         out[c] = f;
@@ -450,6 +451,7 @@ extern "C" int dsim_substrate_create_device(const dsim_mesh *mesh, const dsim_gr
             return fail(DSIM_ERR_ARG, "grid too large");
         }
         g.eps = grid->eps;
+        g.exact_rings = dsim_host::clearance_rings(grid);
     } else {
         delete S;
         return fail(DSIM_ERR_ARG, "unknown grid kind %u", grid->kind);

@@ -216,7 +216,7 @@ inline bool tri_cell_separated(const float *V, const uint32_t *T, uint32_t t, co
 // to any triangle registered in the grid.  A breadth-first pass over the
 // 26-neighbourhood gives each cell its Chebyshev ring distance D to the
 // nearest cell holding a triangle; every triangle is then at least
-// (D - 1) * (smallest cell edge) away.  Cells within kExactRings of a
+// (D - 1) * (smallest cell edge) away.  Cells within `exact_rings` of a
 // triangle refine that with the smallest box-to-box distance between the
 // cell and the bounding boxes of the triangles listed in the rings around it
 // (growing Chebyshev radius, stopping once a ring is farther than the best
@@ -228,8 +228,7 @@ inline bool tri_cell_separated(const float *V, const uint32_t *T, uint32_t t, co
 template <class Dist2>
 void compute_clearance(const uint32_t dims[3], const float lo[3], const float cell[3],
                        const std::vector<uint32_t> &start, const std::vector<uint32_t> &tri,
-                       Dist2 dist2, std::vector<float> &out) {
-    constexpr uint32_t kExactRings = 3;
+                       Dist2 dist2, int64_t exact_rings, std::vector<float> &out) {
     constexpr uint32_t kFar = 0xFFFFFFFFu;
     const uint64_t nc = (uint64_t)dims[0] * dims[1] * dims[2];
     out.assign(nc, 0.0f);
@@ -262,7 +261,7 @@ void compute_clearance(const uint32_t dims[3], const float lo[3], const float ce
                 // triangles lie only in rings >= ring[c], each at least
                 // (ring - 1) * hmin from this cell
                 double best = ring[c] > 0 ? (double)(ring[c] - 1) * hmin : 0.0;
-                if (ring[c] <= kExactRings) {
+                if ((int64_t)ring[c] <= exact_rings) {
                     int64_t ix = c % D[0], iy = (c / D[0]) % D[1], iz = c / ((uint64_t)D[0] * D[1]);
                     int64_t idx[3] = {ix, iy, iz};
                     double b0[3], b1[3];
@@ -322,6 +321,13 @@ int dsim_host::check_mesh(const dsim_mesh *mesh) {
         if (mesh->tri_info[t * 3 + 2] >= nV) return fail(DSIM_ERR_ARG, "triangle %u P1 bad", t);
     }
     return DSIM_OK;
+}
+
+// Rings of exact clearance refinement (cells at ring distance <= this):
+// dsim_grid_params.clearance_rings - 2, or the grid kind's default.
+int64_t dsim_host::clearance_rings(const dsim_grid_params *g) {
+    if (g->clearance_rings == 0) return g->kind == DSIM_GRID_TRIBOX ? kTriboxRings : 3;
+    return (int64_t)g->clearance_rings - 2;
 }
 
 // The axis of the physical mode's labelling rays.  A ray that runs along an
@@ -518,7 +524,7 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
                               }
                               return d2;
                           },
-                          S->clearance);
+                          dsim_host::clearance_rings(grid), S->clearance);
     } else if (grid->kind == DSIM_GRID_TRIBOX) {
         compute_clearance(S->dims, S->lo, S->cell, S->cell_start, S->cell_tri,
                           [&](uint32_t t, const double b0[3], const double b1[3]) {
@@ -528,7 +534,7 @@ extern "C" int dsim_substrate_create(const dsim_mesh *mesh, const dsim_grid_para
                               dsim_host::tribox_setup(a, b, c, b0, b1, X);
                               return dsim_host::tribox_gap2(X);
                           },
-                          S->clearance);
+                          dsim_host::clearance_rings(grid), S->clearance);
     }
     S->label_axis = dsim_host::label_axis_of(mesh);
 

@@ -20,6 +20,15 @@ int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int check_mesh(const dsim_mesh *mesh);
 // Axis (0 x, 1 y, 2 z) of the physical mode's labelling rays for a mesh.
 uint32_t label_axis_of(const dsim_mesh *mesh);
+// Exact clearance refinement of exact-membership grids by default: rings 0-1.
+// Their exact distances cost a separating-axis gap per listed triangle of the
+// surrounding cells; measured on crossing substrates, refining rings 0-1 keeps
+// the walk within 1% of refining 0-3 at a third of the build time, and the
+// ring distance alone costs 6-17% of the walk.
+constexpr int64_t kTriboxRings = 1;
+// Rings of exact clearance refinement of a grid (cells at ring distance at
+// most this; -1 none).
+int64_t clearance_rings(const dsim_grid_params *g);
 
 // The largest float whose square is at most x >= 0 (the float floor of the
 // square root, whatever the rounding of the double square root: the square of

@@ -295,12 +295,16 @@ __device__ __forceinline__ bool better(const Hit &c, const Hit &b) {
     return (c.d < b.d) || (c.d == b.d && c.idx < b.idx);
 }
 
-// Moller-Trumbore vertices of grid entry k and its triangle index: from the
-// cell-major copy, or (exact-membership grids keep none) from the
-// triangle's own record through cell_tri.  The branch is uniform.
+// Where the walk reads a grid entry's records: kRecsCellMajor from the
+// cell-major copies (AABB and reference grids), kRecsIndirect from the
+// triangle's own record through cell_tri (exact-membership grids keep no
+// copy), kRecsEither deciding per launch parameter (a uniform branch; the
+// batched kernel, whose jobs may use either kind of grid).
+// Moller-Trumbore vertices of grid entry k and its triangle index.
+template <int RECS>
 __device__ __forceinline__ uint32_t entry_mt(const StepParams &P, uint32_t k, float4 &v0,
                                              float4 &e1, float4 &e2) {
-    if (P.cell_mt) {
+    if (RECS == kRecsCellMajor || (RECS == kRecsEither && P.cell_mt)) {
         v0 = P.cell_mt[(size_t)k * 3 + 0];
         e1 = P.cell_mt[(size_t)k * 3 + 1];
         e2 = P.cell_mt[(size_t)k * 3 + 2];
@@ -313,8 +317,10 @@ __device__ __forceinline__ uint32_t entry_mt(const StepParams &P, uint32_t k, fl
     return t;
 }
 // Plane (n, n.v0) of grid entry k's triangle.
+template <int RECS>
 __device__ __forceinline__ float4 entry_plane(const StepParams &P, uint32_t k) {
-    return P.cell_plane ? P.cell_plane[k] : P.tri_mt[(size_t)P.cell_tri[k] * 4];
+    if (RECS == kRecsCellMajor || (RECS == kRecsEither && P.cell_plane)) return P.cell_plane[k];
+    return P.tri_mt[(size_t)P.cell_tri[k] * 4];
 }
 
 // cubeCollDetect, spinKernelCpu.cpp:311-337, over one CSR cell.
@@ -333,7 +339,7 @@ __device__ __forceinline__ void cell_test(const StepParams &P, uint32_t cell, ui
         } else {
             // the cell-major plane record rejects most candidates without
             // touching the triangle record
-            if (!crosses_plane(entry_plane(P, k), ox, oy, oz, px, py, pz)) continue;
+            if (!crosses_plane(entry_plane<kRecsEither>(P, k), ox, oy, oz, px, py, pz)) continue;
             uint32_t t = P.cell_tri[k];
             if (t == excluded) continue;
             Hit c;
@@ -663,6 +669,7 @@ __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
 // FP32 triangle test of queue entry q (cell-major record) for its lane; the
 // lane's minimum key in best[] is lowered to it on a hit.  Returns true for a
 // near-grazing entry, which wave_graze_pass decides.
+template <int RECS>
 __device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, uint32_t q,
                                            uint32_t qlen, Hit &h, uint32_t &owner,
                                            unsigned long long &key) {
@@ -673,7 +680,7 @@ __device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, 
     const uint2 e = L.queue[q];
     owner = e.y;
     float4 v0, e1, e2;
-    const uint32_t t = entry_mt(P, e.x, v0, e1, e2);
+    const uint32_t t = entry_mt<RECS>(P, e.x, v0, e1, e2);
     const float4 so = L.o[owner], sp = L.p[owner];
     if (t == __float_as_uint(so.w)) return false;
     if (mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) return true;
@@ -687,6 +694,7 @@ __device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, 
 // FP64 re-test of the near-grazing queue entries in [0, qlen) (found again
 // by the FP32 test), one entry per lane per trip, with the nearest-hit
 // bookkeeping of the FP32 pass.  Wave-uniform qlen.
+template <int RECS>
 __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch &L,
                                                 uint32_t lane, uint32_t qlen) {
     for (uint32_t qb = 0; qb < qlen; qb += 64) {
@@ -699,7 +707,7 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
         if (q < qlen) {
             const uint2 e = L.queue[q];
             float4 v0, e1, e2;
-            const uint32_t t = entry_mt(P, e.x, v0, e1, e2);
+            const uint32_t t = entry_mt<RECS>(P, e.x, v0, e1, e2);
             const float4 so = L.o[e.y], sp = L.p[e.y];
             if (t != __float_as_uint(so.w) &&
                 mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) {
@@ -719,6 +727,7 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
 
 // Triangle pass over queue[0, qlen): nearest hit per lane into best[] and its
 // point into hit[].  Wave-uniform qlen.
+template <int RECS>
 __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScratch &L,
                                                    uint32_t lane, uint32_t qlen) {
     bool graze = false;
@@ -727,8 +736,8 @@ __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScra
         Hit ha, hb;
         uint32_t oa, ob;
         unsigned long long ka, kb;
-        graze |= queue_test(P, L, qb + lane, qlen, ha, oa, ka);
-        graze |= queue_test(P, L, qb + 64 + lane, qlen, hb, ob, kb);
+        graze |= queue_test<RECS>(P, L, qb + lane, qlen, ha, oa, ka);
+        graze |= queue_test<RECS>(P, L, qb + 64 + lane, qlen, hb, ob, kb);
         wave_lds_sync();
         // the pair holding the lane's minimum so far records its hit point; a
         // later pass that lowers the minimum overwrites it
@@ -737,12 +746,13 @@ __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScra
         if (hb.type && L.best[ob] == kb)
             L.hit[ob] = make_float4(hb.px, hb.py, hb.pz, __uint_as_float(hb.type));
     }
-    if (__builtin_expect(__ballot(graze) != 0, 0)) wave_graze_pass(P, L, lane, qlen);
+    if (__builtin_expect(__ballot(graze) != 0, 0)) wave_graze_pass<RECS>(P, L, lane, qlen);
     wave_lds_sync();
 }
 
 // Plane test of pair j for its owner lane: sets k (grid entry) and whether
 // the owner's segment crosses the entry's plane.
+template <int RECS>
 __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScratch &L,
                                              uint32_t j, uint32_t total, uint32_t owner,
                                              uint32_t &k) {
@@ -759,7 +769,7 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
         k = 0;
     }
     const float4 so = L.o[owner], sp = L.p[owner];
-    return crosses_plane(entry_plane(P, k), so.x, so.y, so.z, sp.x, sp.y, sp.z);
+    return crosses_plane(entry_plane<RECS>(P, k), so.x, so.y, so.z, sp.x, sp.y, sp.z);
 }
 
 // One search round, called by all 64 lanes of the wave together.  For each
@@ -771,6 +781,7 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
 // A box of more than kWaveSegs cells is searched kWaveSegs cells per round, in
 // x-fastest order from cell `slice` (0 for a new segment; advanced here), its
 // nearest hit carried in best[]/hit[] from round to round.
+template <int RECS>
 __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L, uint32_t lane,
                                             bool search, float ox, float oy, float oz, float px,
                                             float py, float pz, uint32_t excluded,
@@ -845,19 +856,19 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
         const uint32_t own_b = max(wave_incl_max(L.map[lane + 64]),
                                    __builtin_amdgcn_readlane(own_a, 63));
         uint32_t ka, kb;
-        const bool ca = pair_crosses(P, L, base + lane, total, own_a - 1, ka);
-        const bool cb = pair_crosses(P, L, base + 64 + lane, total, own_b - 1, kb);
+        const bool ca = pair_crosses<RECS>(P, L, base + lane, total, own_a - 1, ka);
+        const bool cb = pair_crosses<RECS>(P, L, base + 64 + lane, total, own_b - 1, kb);
         const unsigned long long ba = __ballot(ca), bb = __ballot(cb);
         if (ca) L.queue[qlen + lane_rank(ba)] = make_uint2(ka, own_a - 1);
         qlen += (uint32_t)__popcll(ba);
         if (cb) L.queue[qlen + lane_rank(bb)] = make_uint2(kb, own_b - 1);
         qlen += (uint32_t)__popcll(bb);
         if (qlen > kWaveQueue - kPass) {
-            wave_triangle_pass(P, L, lane, qlen);
+            wave_triangle_pass<RECS>(P, L, lane, qlen);
             qlen = 0;
         }
     }
-    if (qlen) wave_triangle_pass(P, L, lane, qlen);
+    if (qlen) wave_triangle_pass<RECS>(P, L, lane, qlen);
     wave_lds_sync();
     key = ~0ull;
     const bool done = coop && slice + nreq >= ncell;
@@ -904,9 +915,14 @@ __device__ __forceinline__ uint32_t pack_wrap(int kx, int ky, int kz) {
 
 // cpuIntegrate (spinKernelCpu.cpp:430-499) in physical mode: `iters` steps
 // for the blockDim.x spins of block `block` of the simulation described by P.
-template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+// Boundary of the walk: clamp box, the periodic cube [-1, 1)^3 (constant
+// bounds), or a periodic box with the bounds of the launch parameters.
+enum { kBoundClamp = 0, kBoundCube = 1, kBoundBox = 2 };
+
+template <int BOUND, int RNG, bool TRACE, bool MOMENT, int RECS>
 __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
                                           WaveScratch &L) {
+    constexpr bool PERIODIC = BOUND != kBoundClamp;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t i = block * blockDim.x + threadIdx.x;
     // lanes past the last spin stay in the loop (done) for the wave operations
@@ -978,7 +994,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         unsigned long long key;
         float4 hp;
         const bool searched =
-            wave_search(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, slice, key, hp);
+            wave_search<RECS>(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, slice, key, hp);
         if (pending && searched) {
             if (key == ~0ull) {
                 pending = false;
@@ -1024,9 +1040,15 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             }
             if (PERIODIC) {
                 const float wx = px, wy = py, wz = pz;
-                wrap_axis(px, P.wrap_lo[0], P.wrap_hi[0], P.wrap_len[0], kx);
-                wrap_axis(py, P.wrap_lo[1], P.wrap_hi[1], P.wrap_len[1], ky);
-                wrap_axis(pz, P.wrap_lo[2], P.wrap_hi[2], P.wrap_len[2], kz);
+                if (BOUND == kBoundCube) {
+                    wrap_axis(px, -1.0f, 1.0f, 2.0f, kx);
+                    wrap_axis(py, -1.0f, 1.0f, 2.0f, ky);
+                    wrap_axis(pz, -1.0f, 1.0f, 2.0f, kz);
+                } else {
+                    wrap_axis(px, P.wrap_lo[0], P.wrap_hi[0], P.wrap_len[0], kx);
+                    wrap_axis(py, P.wrap_lo[1], P.wrap_hi[1], P.wrap_len[1], ky);
+                    wrap_axis(pz, P.wrap_lo[2], P.wrap_hi[2], P.wrap_len[2], kz);
+                }
                 refresh |= (px != wx) | (py != wy) | (pz != wz);
             }
             x = px; y = py; z = pz;
@@ -1039,9 +1061,12 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             // for a spin that has not crossed a face)
             if (!MOMENT && !PERIODIC) ph += (P.gx * x + P.gy * y + P.gz * z) * P.pc;
             if (!MOMENT && PERIODIC) {
-                const float ux = x + (float)kx * P.wrap_len[0];
-                const float uy = y + (float)ky * P.wrap_len[1];
-                const float uz = z + (float)kz * P.wrap_len[2];
+                const float lx = BOUND == kBoundCube ? 2.0f : P.wrap_len[0];
+                const float ly = BOUND == kBoundCube ? 2.0f : P.wrap_len[1];
+                const float lz = BOUND == kBoundCube ? 2.0f : P.wrap_len[2];
+                const float ux = x + (float)kx * lx;
+                const float uy = y + (float)ky * ly;
+                const float uz = z + (float)kz * lz;
                 ph += (P.gx * ux + P.gy * uy + P.gz * uz) * P.pc;
             }
             if (MOMENT && P.weight != 0.0f) {
@@ -1084,14 +1109,14 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     }
 }
 
-template <bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+template <int BOUND, int RNG, bool TRACE, bool MOMENT, int RECS>
 // One wave per workgroup: every wave runs until its own slowest lane is done,
 // and a finished wave frees its slot for the next block at once (with four
 // waves per workgroup the slot waited for the slowest of the four).
 __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
     StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[1];
-    walk_body<PERIODIC, RNG, TRACE, MOMENT>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
+    walk_body<BOUND, RNG, TRACE, MOMENT, RECS>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
 }
 
 // Batched walk for the server: the blocks of one launch are shared out over
@@ -1101,7 +1126,7 @@ __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu
 // exactly the arithmetic of a k_walk launch of their own.  Blocks past the
 // table's total exit at once, so one launch shape (and one captured graph)
 // serves any mix of jobs.
-template <bool PERIODIC>
+template <int BOUND, int RECS>
 __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_batch(const WalkTable *__restrict__ T) {
     __shared__ WaveScratch scratch[1];
     const uint32_t b = xcd_block(kBatchXcdRun);
@@ -1110,61 +1135,75 @@ __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu
     for (uint32_t st = kMaxJobs / 2; st; st >>= 1)
         if (j + st < T->n_jobs && T->block_start[j + st] <= b) j += st;
     const WalkJob &J = T->job[j];
-    walk_body<PERIODIC, kRngPhilox, false, true>(J.p, b - T->block_start[j], J.iters, scratch[0]);
+    walk_body<BOUND, kRngPhilox, false, true, RECS>(J.p, b - T->block_start[j], J.iters,
+                                                    scratch[0]);
 }
 
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
-                             hipStream_t s) {
+                             int recs, hipStream_t s) {
     if (grid_blocks == 0) return hipSuccess;
-    if (periodic)
-        hipLaunchKernelGGL(k_walk_batch<true>, dim3(grid_blocks), dim3(kWalkBlockSpins), 0, s, table);
-    else
-        hipLaunchKernelGGL(k_walk_batch<false>, dim3(grid_blocks), dim3(kWalkBlockSpins), 0, s,
-                           table);
+    const dim3 g(grid_blocks), b(kWalkBlockSpins);
+    // every job's voxel comes from its parameters (kBoundBox): jobs differ
+    if (periodic) {
+        if (recs == kRecsCellMajor) hipLaunchKernelGGL((k_walk_batch<kBoundBox, kRecsCellMajor>), g, b, 0, s, table);
+        else if (recs == kRecsIndirect) hipLaunchKernelGGL((k_walk_batch<kBoundBox, kRecsIndirect>), g, b, 0, s, table);
+        else hipLaunchKernelGGL((k_walk_batch<kBoundBox, kRecsEither>), g, b, 0, s, table);
+    } else {
+        if (recs == kRecsCellMajor) hipLaunchKernelGGL((k_walk_batch<kBoundClamp, kRecsCellMajor>), g, b, 0, s, table);
+        else if (recs == kRecsIndirect) hipLaunchKernelGGL((k_walk_batch<kBoundClamp, kRecsIndirect>), g, b, 0, s, table);
+        else hipLaunchKernelGGL((k_walk_batch<kBoundClamp, kRecsEither>), g, b, 0, s, table);
+    }
     return hipGetLastError();
 }
 
-template <int MODE, bool PERIODIC, int RNG, bool TRACE, bool MOMENT>
+template <int MODE, int BOUND, int RNG, bool TRACE, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     if constexpr (MODE == 0) {
-        hipLaunchKernelGGL((k_step<0, PERIODIC, RNG, TRACE, MOMENT>), dim3((p.n + 63) / 64),
-                           dim3(64), 0, s, p, iters);
+        hipLaunchKernelGGL((k_step<0, BOUND != kBoundClamp, RNG, TRACE, MOMENT>),
+                           dim3((p.n + 63) / 64), dim3(64), 0, s, p, iters);
+    } else if (p.cell_mt) {
+        hipLaunchKernelGGL((k_walk<BOUND, RNG, TRACE, MOMENT, kRecsCellMajor>),
+                           dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
+                           dim3(kWalkBlockSpins), 0, s, p, iters);
     } else {
-        hipLaunchKernelGGL((k_walk<PERIODIC, RNG, TRACE, MOMENT>),
+        hipLaunchKernelGGL((k_walk<BOUND, RNG, TRACE, MOMENT, kRecsIndirect>),
                            dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
                            dim3(kWalkBlockSpins), 0, s, p, iters);
     }
 }
 
-template <int MODE, bool PERIODIC, int RNG>
+template <int MODE, int BOUND, int RNG>
 static hipError_t launch3(const StepParams &p, uint32_t iters, bool trace, bool moment,
                           hipStream_t s) {
     if (trace) {
-        if (moment) launch_one<MODE, PERIODIC, RNG, true, true>(p, iters, s);
-        else launch_one<MODE, PERIODIC, RNG, true, false>(p, iters, s);
+        if (moment) launch_one<MODE, BOUND, RNG, true, true>(p, iters, s);
+        else launch_one<MODE, BOUND, RNG, true, false>(p, iters, s);
     } else {
-        if (moment) launch_one<MODE, PERIODIC, RNG, false, true>(p, iters, s);
-        else launch_one<MODE, PERIODIC, RNG, false, false>(p, iters, s);
+        if (moment) launch_one<MODE, BOUND, RNG, false, true>(p, iters, s);
+        else launch_one<MODE, BOUND, RNG, false, false>(p, iters, s);
     }
     return hipGetLastError();
 }
 
-template <int MODE, bool PERIODIC>
+template <int MODE, int BOUND>
 static hipError_t launch2(const StepParams &p, uint32_t iters, int rng, bool trace, bool moment,
                           hipStream_t s) {
     switch (rng) {
-        case kRngInject: return launch3<MODE, PERIODIC, kRngInject>(p, iters, trace, moment, s);
-        case kRngMwc: return launch3<MODE, PERIODIC, kRngMwc>(p, iters, trace, moment, s);
-        default: return launch3<MODE, PERIODIC, kRngPhilox>(p, iters, trace, moment, s);
+        case kRngInject: return launch3<MODE, BOUND, kRngInject>(p, iters, trace, moment, s);
+        case kRngMwc: return launch3<MODE, BOUND, kRngMwc>(p, iters, trace, moment, s);
+        default: return launch3<MODE, BOUND, kRngPhilox>(p, iters, trace, moment, s);
     }
 }
 
 hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
                        bool trace, bool moment, hipStream_t s) {
     if (p.n == 0 || iters == 0) return hipSuccess;
-    if (mode == 0) return launch2<0, false>(p, iters, rng, trace, moment, s);
-    if (periodic) return launch2<1, true>(p, iters, rng, trace, moment, s);
-    return launch2<1, false>(p, iters, rng, trace, moment, s);
+    if (mode == 0) return launch2<0, kBoundClamp>(p, iters, rng, trace, moment, s);
+    if (!periodic) return launch2<1, kBoundClamp>(p, iters, rng, trace, moment, s);
+    bool cube = true;
+    for (int k = 0; k < 3; ++k) cube &= p.wrap_lo[k] == -1.0f && p.wrap_hi[k] == 1.0f;
+    if (cube) return launch2<1, kBoundCube>(p, iters, rng, trace, moment, s);
+    return launch2<1, kBoundBox>(p, iters, rng, trace, moment, s);
 }
 
 // ------------------------------------------------------------------------ //
@@ -1904,11 +1943,12 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
 __global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, float *y, float *z,
                                                       float *mag, float *phase, uint32_t *label,
                                                       uint32_t *id, uint32_t *wrap, float3 width,
-                                                      float3 centre, uint32_t k0, uint32_t k1) {
+                                                      float3 centre, uint32_t k0, uint32_t k1,
+                                                      uint32_t d0, uint32_t d1) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     id[i] = i;
-    uint4 o = philox(make_uint4(i, 0u, 0u, 2u), k0, k1);
+    uint4 o = philox(make_uint4(i, d0, d1, 2u), k0, k1);
     x[i] = width.x * (u32_unit(o.x) - 0.5f) + centre.x;
     y[i] = width.y * (u32_unit(o.y) - 0.5f) + centre.y;
     z[i] = width.z * (u32_unit(o.z) - 0.5f) + centre.z;
@@ -1920,13 +1960,15 @@ __global__ __launch_bounds__(256) void k_init_uniform(uint32_t n, float *x, floa
 
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
                                uint32_t *label, uint32_t *id, uint32_t *wrap, const float lo[3],
-                               const float hi[3], uint32_t key0, uint32_t key1, hipStream_t s) {
+                               const float hi[3], uint32_t key0, uint32_t key1, uint64_t draw,
+                               hipStream_t s) {
     if (n == 0) return hipSuccess;
     const float3 w = make_float3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
     const float3 c = make_float3(0.5f * (hi[0] + lo[0]), 0.5f * (hi[1] + lo[1]),
                                  0.5f * (hi[2] + lo[2]));
     hipLaunchKernelGGL(k_init_uniform, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, mag,
-                       phase, label, id, wrap, w, c, key0, key1);
+                       phase, label, id, wrap, w, c, key0, key1, (uint32_t)draw,
+                       (uint32_t)(draw >> 32));
     return hipGetLastError();
 }
 

@@ -57,8 +57,9 @@ struct Req {
     dsim_request_stats st{};
 };
 
-// One launch kind (boundary) of the batched step: its job table in pinned
-// host memory and on the device, and the captured launch.
+// One launch kind of the batched step (boundary, and whether the jobs' grids
+// keep cell-major records): its job table in pinned host memory and on the
+// device, and the captured launch.
 struct Batch {
     dsim::WalkTable *host = nullptr;
     dsim::WalkTable *dev = nullptr;
@@ -84,7 +85,7 @@ struct dsim_server {
     bool running = false, stopping = false;
     int loop_err = DSIM_OK;
     hipStream_t stream = nullptr;
-    Batch batch[2];  // [boundary]
+    Batch batch[4];  // [boundary * 2 + (grid entries read through cell_tri)]
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     double clock() const {
@@ -173,8 +174,9 @@ int ensure_batch(dsim_server *S, int kind) {
     // the launch shape never changes: blocks past the table's total exit at
     // once, so one captured launch serves every round
     HIP_TRY(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = dsim::launch_walk_batch(B.dev, S->grid_blocks, kind == DSIM_BOUNDARY_PERIODIC,
-                                           S->stream);
+    hipError_t e = dsim::launch_walk_batch(
+        B.dev, S->grid_blocks, (kind >> 1) == DSIM_BOUNDARY_PERIODIC,
+        (kind & 1) ? dsim::kRecsIndirect : dsim::kRecsCellMajor, S->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(S->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess)
@@ -191,18 +193,20 @@ int ensure_batch(dsim_server *S, int kind) {
 // for the step; (4) advance the bookkeeping; (5) release finished requests.
 int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
     std::lock_guard<std::mutex> rl(S->round_mu);
-    // (1) one batched launch per boundary kind carries every prepared request.
+    // (1) one batched launch per kind (boundary, record layout of the grid)
+    // carries every prepared request of that kind.
     // A round lasts about total_blocks / kResidentBlocks times as long as one
     // block's round_steps steps, so a job of few blocks can walk that many
     // more steps in it without making it longer: small jobs get longer chunks
     // (fewer rounds, lower latency) and go first in the table, so their
     // blocks are dispatched first.
     bool launched = false;
-    for (int kind = 0; kind < 2; ++kind) {
+    for (int kind = 0; kind < 4; ++kind) {
         std::vector<Req *> jobs;
         uint32_t total = 0;
         for (Req *r : S->active) {
-            if (!r->ready || r->read_out || r->r.boundary != (uint32_t)kind || r->lobe_left == 0)
+            if (!r->ready || r->read_out || r->r.boundary != (uint32_t)(kind >> 1) ||
+                r->lobe_left == 0 || (r->sub->cell_mt.p == nullptr) != ((kind & 1) != 0))
                 continue;
             jobs.push_back(r);
             total += walk_blocks(r->r.n_spins);

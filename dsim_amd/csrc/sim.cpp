@@ -593,7 +593,7 @@ extern "C" int dsim_init_spins_box(dsim_sim *s, const float lo[3], const float h
     hipError_t e = dsim::launch_init_uniform(s->n, s->ff(F_X), s->ff(F_Y), s->ff(F_Z), s->ff(F_MAG),
                                              s->ff(F_PHASE), s->f(F_LABEL), s->f(F_ID),
                                              s->f(F_WRAP), lo, hi, (uint32_t)s->prm.seed,
-                                             (uint32_t)(s->prm.seed >> 32), s->stream);
+                                             (uint32_t)(s->prm.seed >> 32), s->step, s->stream);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
     s->permuted = false;
     s->moment_ready = false;

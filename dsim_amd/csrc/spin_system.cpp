@@ -74,16 +74,31 @@ bool SpinSystem::build() {
 
 void SpinSystem::resetSpins() {
     if (!m_sim) return;
-    // new spins for every reset: the Philox key moves with the reset count
-    dsim_set_step(m_sim, 0);
+    // new spins for every reset: the position draw takes the handle's step
+    // counter, set to the reset count
+    dsim_set_step(m_sim, m_resets);
     if (!check(dsim_init_spins_uniform(m_sim, m_startBox), "resetSpins")) return;
+    dsim_set_step(m_sim, 0);
+    findSpinsInFibers();
+    assignSeeds();
+    m_resets++;
+}
+
+void SpinSystem::findSpinsInFibers() {
+    if (!m_sim) return;
     uint32_t amb = 0;
     if (!check(dsim_label_spins(m_sim, m_info.xmax, m_info.ymax, m_info.zmax, 20, &amb),
                "findSpinsInFibers"))
         return;
     if (amb) fprintf(stderr, "findSpinsInFibers: %u spins stayed on membrane edges\n", amb);
-    assignSeeds();
-    m_resets++;
+}
+
+bool SpinSystem::setArray(const float *pos, const dsim_spin_data *spins, const uint32_t *seeds) {
+    return m_sim && check(dsim_set_state_aos(m_sim, pos, spins, seeds), "setArray");
+}
+
+bool SpinSystem::getArray(float *pos, dsim_spin_data *spins, uint32_t *seeds) {
+    return m_sim && check(dsim_get_state_aos(m_sim, pos, spins, seeds), "getArray");
 }
 
 void SpinSystem::assignSeeds() {

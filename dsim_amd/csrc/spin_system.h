@@ -35,6 +35,12 @@ class SpinSystem {
     bool build();                             // spinSystem.cpp:178-281
     void resetSpins();                        // spinSystem.cpp:1386-1407
     void assignSeeds();                       // spinSystem.cpp:1532-1537
+    void findSpinsInFibers();                 // spinSystem.cpp:1615-1852
+    // setArray + setSpinArray (spinSystem.cpp:1584-1606) and getArray +
+    // getSpinArray (spinSystem.cpp:1544-1571): host AoS positions [n*3],
+    // spinData [n], MWC seeds [n*2]; any pointer may be null.
+    bool setArray(const float *pos, const dsim_spin_data *spins, const uint32_t *seeds);
+    bool getArray(float *pos, dsim_spin_data *spins, uint32_t *seeds);
     float updateSpins(float deltaTime, unsigned iterations);  // spinSystem.cpp:1926-1971
     void setGradient(Float3 g) { m_gradient = g; }            // spinSystem.h:163
     double getMrSignal();                                      // spinSystem.cpp:1418-1450

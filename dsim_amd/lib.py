@@ -51,7 +51,7 @@ class Mesh(C.Structure):
 
 class GridParams(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("dims", C.c_uint32 * 3), ("lo", C.c_float * 3),
-                ("cell", C.c_float * 3), ("eps", C.c_float)]
+                ("cell", C.c_float * 3), ("eps", C.c_float), ("clearance_rings", C.c_uint32)]
 
 
 class Params(C.Structure):
@@ -185,7 +185,7 @@ class Substrate:
     the arrays equal the host build's."""
 
     def __init__(self, vertices, triangles, tri_info, *, grid="reference", n_cubes=50, dims=None,
-                 lo=None, cell=None, eps=0.0, device=False):
+                 lo=None, cell=None, eps=0.0, device=False, clearance_rings=0):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.triangles = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
         self.tri_info = np.ascontiguousarray(tri_info, np.uint32).reshape(-1, 3)
@@ -201,6 +201,7 @@ class Substrate:
             g.lo = (C.c_float * 3)(*[float(x) for x in lo])
             g.cell = (C.c_float * 3)(*[float(x) for x in cell])
             g.eps = float(eps)
+            g.clearance_rings = int(clearance_rings)
         h = _vp()
         create = lib().dsim_substrate_create_device if device else lib().dsim_substrate_create
         check(create(C.byref(m), C.byref(g), C.byref(h)))

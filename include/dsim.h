@@ -77,6 +77,12 @@ typedef struct {
     float lo[3];      /* AABB: grid origin (normalised units) */
     float cell[3];    /* AABB: cell edge per axis */
     float eps;        /* AABB: membership margin */
+    uint32_t clearance_rings; /* AABB and exact grids: cells at ring distance at most
+                               * clearance_rings - 2 from a listed triangle refine their
+                               * clearance with exact distances, farther ones use the
+                               * ring distance alone (1: none refine; 0: the default,
+                               * rings 0-3 for AABB grids and rings 0-1 for exact
+                               * grids) */
 } dsim_grid_params;
 
 enum {
@@ -189,7 +195,9 @@ int dsim_get_step(dsim_sim *sim, uint64_t *step);
 
 /* resetSpins' position draw (spinSystem.cpp:1386-1398), on the device:
  * positions uniform in [-start_box, start_box]^3 from the Philox stream,
- * magnitude 1, phase 0, labels cleared.  Follow with dsim_label_spins. */
+ * magnitude 1, phase 0, labels cleared.  The draw is keyed by the handle's
+ * seed and its step counter (dsim_set_step), so draws at different counter
+ * values differ.  Follow with dsim_label_spins. */
 int dsim_init_spins_uniform(dsim_sim *sim, float start_box);
 /* The same draw over a box [lo, hi] per axis (a non-cubic periodic voxel):
  * coordinate = (hi - lo) * (u - 0.5) + (hi + lo) / 2, u from the same Philox

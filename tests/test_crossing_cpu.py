@@ -115,10 +115,12 @@ def _point_tri_dist(p, a, b, c):
     return best
 
 
-def test_tribox_clearance_is_a_lower_bound():
+@pytest.mark.parametrize("rings", [0, 5])
+def test_tribox_clearance_is_a_lower_bound(rings):
+    # the default (ring distance only) and exact refinement of rings 0-3
     cx, V, T, I, s = _small(63.4, n=20)
     g = X.grid_params(cx, 1.5, 1.5)
-    sub = D.Substrate(V, T, I, grid="tribox", **g)
+    sub = D.Substrate(V, T, I, grid="tribox", clearance_rings=rings, **g)
     cl = sub.clearance()
     dims, lo, cell = g["dims"], np.array(g["lo"]), np.array(g["cell"])
     rng = np.random.default_rng(7)
