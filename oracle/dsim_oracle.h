@@ -203,13 +203,16 @@ typedef struct {
     uint32_t *trgls;    /* malloc'd */
     uint32_t *tri_info; /* malloc'd */
 } orc_mesh;
-/* 0 ok, -1 file missing, -2 axon/myelin count mismatch (spinSystem.cpp:593-596) */
+/* 0 ok, -1 file missing, -2 axon/myelin count mismatch (spinSystem.cpp:593-596).
+ * On -2 the arrays are still filled; release them with orc_free_mesh after
+ * any return. */
 int orc_parse_mesh(const char *path, float space_scale, orc_mesh *m);
 void orc_free_mesh(orc_mesh *m);
 /* Returns number of gradient lines, fills up to cap rows of 6 floats. */
 int orc_parse_grads(const char *path, float *rows, int cap);
 /* Per-update gradient sign schedule for one line: sign[k] in {+1,0,-1} for
- * updates k=0..n-1, readout at update n.  Returns n = L+B+R. */
+ * updates k=0..n-1, readout at update n.  Returns n = L+B+R; lbr (4 ints,
+ * or NULL) receives L, B, R and whether the readout happens. */
 int orc_pgse_schedule(float ldelta, float bdelta, float readout, float dt,
                       int steps_per_update, int *sign, int cap, int *lbr);
 
