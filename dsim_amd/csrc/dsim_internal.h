@@ -66,6 +66,7 @@ struct StepParams {
     const float4 *inject;  // [iters][n]
     uint32_t *trace;       // [iters][n][5]
     uint32_t *flags;       // status word: bit 0 collision loop guard hit, bit 2 entry out of range
+    unsigned long long *stats;  // [kWalkStats] work counters (counting build of the walk)
 };
 
 struct LabelParams {
@@ -97,6 +98,16 @@ struct LabelParams {
 
 enum Rng { kRngInject = 0, kRngMwc = 1, kRngPhilox = 2 };
 
+// What a step launch records beside the step: nothing, the per-step trace,
+// or the walk's work counters (the DSIM_STAT_* indices of dsim.h).
+enum Instr { kInstrNone = 0, kInstrTrace = 1, kInstrStats = 2 };
+constexpr uint32_t kWalkStats = 16;
+enum WalkStat {
+    kStatSteps = 0, kStatTrips, kStatSearchLanes, kStatCells, kStatPlane, kStatPasses,
+    kStatTriangle, kStatTriPasses, kStatGraze, kStatHits, kStatShortcut, kStatDeferred,
+    kStatWaves, kStatLiveLaneTrips
+};
+
 // Job table of the batched walk (physical mode, Philox, moment accumulation):
 // job j advances the simulation of job[j].p by job[j].iters steps on blocks
 // [block_start[j], block_start[j+1]).
@@ -118,7 +129,7 @@ struct WalkTable {
 
 // Launchers (kernels.hip).  Return hipError_t of the launch.
 hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
-                       bool trace, bool moment, hipStream_t s);
+                       int instr, bool moment, hipStream_t s);
 hipError_t launch_signal(uint32_t n, const float *mag, const float *phase, const uint32_t *label,
                          double *partial, uint32_t n_blocks, double *out15, hipStream_t s);
 hipError_t launch_moment_signal(uint32_t n, const float *mag, const float *mom, const float *x0,

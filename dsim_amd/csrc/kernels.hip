@@ -7,7 +7,6 @@
 // physical-mode code spells its fused multiply-adds out with fmaf().
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/warp/warp_scan.hpp>
 
 #include <cstdint>
 
@@ -642,21 +641,32 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-using WaveScan = rocprim::warp_scan<uint32_t, 64>;  // DPP row shifts and broadcasts
-
-// Inclusive prefix sum / maximum over the wave; every lane must be active.
-__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
-    typename WaveScan::storage_type st;
-    uint32_t out;
-    WaveScan().inclusive_scan(v, out, st, rocprim::plus<uint32_t>());
-    return out;
+// Inclusive scan over the 64 lanes for an operation whose identity is 0
+// (sum, unsigned maximum); every lane must be active.  Four DPP row shifts
+// (1, 2, 4, 8) scan each row of 16 lanes; row_bcast:15 then carries row 0
+// into row 1 and row 2 into row 3 (row mask 0b1010), and row_bcast:31
+// carries lane 31 into rows 2 and 3 (row mask 0b1100).  A lane whose source
+// is outside its row, or whose row is masked off, reads the identity, so each
+// step is one VALU operation with a DPP operand.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
+#define DSIM_SCAN_STEP(ctrl, rows)                                                      \
+    {                                                                                  \
+        const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, rows, 0xf, \
+                                                                 false);               \
+        v = MAX ? max(v, t) : v + t;                                                   \
+    }
+    DSIM_SCAN_STEP(0x111, 0xf)  // row_shr:1
+    DSIM_SCAN_STEP(0x112, 0xf)  // row_shr:2
+    DSIM_SCAN_STEP(0x114, 0xf)  // row_shr:4
+    DSIM_SCAN_STEP(0x118, 0xf)  // row_shr:8
+    DSIM_SCAN_STEP(0x142, 0xa)  // row_bcast:15
+    DSIM_SCAN_STEP(0x143, 0xc)  // row_bcast:31
+#undef DSIM_SCAN_STEP
+    return v;
 }
-__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
-    typename WaveScan::storage_type st;
-    uint32_t out;
-    WaveScan().inclusive_scan(v, out, st, rocprim::maximum<uint32_t>());
-    return out;
-}
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) { return wave_scan<false>(v); }
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) { return wave_scan<true>(v); }
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -665,6 +675,14 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
 __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t tri) {
     return ((unsigned long long)__float_as_uint(t) << 32) | tri;
 }
+
+// Work counters of one wave in the counting build of the walk (indices
+// kStat*), added to P.stats when the wave ends.  Every count is wave-uniform
+// where it is taken.
+struct WalkCounters {
+    uint32_t v[kWalkStats];
+};
+__device__ __forceinline__ uint32_t popc(unsigned long long m) { return (uint32_t)__popcll(m); }
 
 // FP32 triangle test of queue entry q (cell-major record) for its lane; the
 // lane's minimum key in best[] is lowered to it on a hit.  Returns true for a
@@ -694,9 +712,9 @@ __device__ __forceinline__ bool queue_test(const StepParams &P, WaveScratch &L, 
 // FP64 re-test of the near-grazing queue entries in [0, qlen) (found again
 // by the FP32 test), one entry per lane per trip, with the nearest-hit
 // bookkeeping of the FP32 pass.  Wave-uniform qlen.
-template <int RECS>
+template <int RECS, bool STATS>
 __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch &L,
-                                                uint32_t lane, uint32_t qlen) {
+                                                uint32_t lane, uint32_t qlen, WalkCounters &C) {
     for (uint32_t qb = 0; qb < qlen; qb += 64) {
         wave_lds_sync();
         Hit h;
@@ -704,13 +722,15 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
         uint32_t owner = 0;
         unsigned long long key = ~0ull;
         const uint32_t q = qb + lane;
+        bool graze = false;
         if (q < qlen) {
             const uint2 e = L.queue[q];
             float4 v0, e1, e2;
             const uint32_t t = entry_mt<RECS>(P, e.x, v0, e1, e2);
             const float4 so = L.o[e.y], sp = L.p[e.y];
-            if (t != __float_as_uint(so.w) &&
-                mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h)) {
+            graze = t != __float_as_uint(so.w) &&
+                    mt_fp32(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, sp.w, h);
+            if (graze) {
                 mt_fp64(v0, e1, e2, t, so.x, so.y, so.z, sp.x, sp.y, sp.z, h);
                 if (h.type) {
                     owner = e.y;
@@ -719,6 +739,7 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
                 }
             }
         }
+        if (STATS) C.v[kStatGraze] += popc(__ballot(graze));
         wave_lds_sync();
         if (h.type && L.best[owner] == key)
             L.hit[owner] = make_float4(h.px, h.py, h.pz, __uint_as_float(h.type));
@@ -727,9 +748,14 @@ __device__ __forceinline__ void wave_graze_pass(const StepParams &P, WaveScratch
 
 // Triangle pass over queue[0, qlen): nearest hit per lane into best[] and its
 // point into hit[].  Wave-uniform qlen.
-template <int RECS>
+template <int RECS, bool STATS>
 __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScratch &L,
-                                                   uint32_t lane, uint32_t qlen) {
+                                                   uint32_t lane, uint32_t qlen,
+                                                   WalkCounters &C) {
+    if (STATS) {
+        C.v[kStatTriangle] += qlen;
+        C.v[kStatTriPasses] += (qlen + kPass - 1) / kPass;
+    }
     bool graze = false;
     for (uint32_t qb = 0; qb < qlen; qb += kPass) {
         wave_lds_sync();
@@ -746,7 +772,7 @@ __device__ __forceinline__ void wave_triangle_pass(const StepParams &P, WaveScra
         if (hb.type && L.best[ob] == kb)
             L.hit[ob] = make_float4(hb.px, hb.py, hb.pz, __uint_as_float(hb.type));
     }
-    if (__builtin_expect(__ballot(graze) != 0, 0)) wave_graze_pass<RECS>(P, L, lane, qlen);
+    if (__builtin_expect(__ballot(graze) != 0, 0)) wave_graze_pass<RECS, STATS>(P, L, lane, qlen, C);
     wave_lds_sync();
 }
 
@@ -781,12 +807,12 @@ __device__ __forceinline__ bool pair_crosses(const StepParams &P, const WaveScra
 // A box of more than kWaveSegs cells is searched kWaveSegs cells per round, in
 // x-fastest order from cell `slice` (0 for a new segment; advanced here), its
 // nearest hit carried in best[]/hit[] from round to round.
-template <int RECS>
+template <int RECS, bool STATS>
 __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L, uint32_t lane,
                                             bool search, float ox, float oy, float oz, float px,
                                             float py, float pz, uint32_t excluded,
                                             uint32_t &slice, unsigned long long &key,
-                                            float4 &hit) {
+                                            float4 &hit, WalkCounters &C) {
     uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0, ncell = 0;
     if (search) {
         uint32_t sx, sy, sz, tx, ty, tz;
@@ -837,6 +863,13 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
     const uint32_t cend = wave_incl_sum(cnt);
     const uint32_t off = cend - cnt;
     const uint32_t total = __builtin_amdgcn_readlane(cend, 63);
+    if (STATS) {
+        C.v[kStatSearchLanes] += popc(__ballot(coop));
+        C.v[kStatDeferred] += popc(__ballot(nreq > 0 && !coop));
+        C.v[kStatCells] += __builtin_amdgcn_readlane(wave_incl_sum(coop ? nreq : 0u), 63);
+        C.v[kStatPlane] += total;
+        C.v[kStatPasses] += (total + kPass - 1) / kPass;
+    }
     if (coop) {
         L.o[lane] = make_float4(ox, oy, oz, __uint_as_float(excluded));
         L.p[lane] = make_float4(px, py, pz, seg_len2(ox, oy, oz, px, py, pz));
@@ -864,11 +897,11 @@ __device__ __forceinline__ bool wave_search(const StepParams &P, WaveScratch &L,
         if (cb) L.queue[qlen + lane_rank(bb)] = make_uint2(kb, own_b - 1);
         qlen += (uint32_t)__popcll(bb);
         if (qlen > kWaveQueue - kPass) {
-            wave_triangle_pass<RECS>(P, L, lane, qlen);
+            wave_triangle_pass<RECS, STATS>(P, L, lane, qlen, C);
             qlen = 0;
         }
     }
-    if (qlen) wave_triangle_pass<RECS>(P, L, lane, qlen);
+    if (qlen) wave_triangle_pass<RECS, STATS>(P, L, lane, qlen, C);
     wave_lds_sync();
     key = ~0ull;
     const bool done = coop && slice + nreq >= ncell;
@@ -919,10 +952,15 @@ __device__ __forceinline__ uint32_t pack_wrap(int kx, int ky, int kz) {
 // bounds), or a periodic box with the bounds of the launch parameters.
 enum { kBoundClamp = 0, kBoundCube = 1, kBoundBox = 2 };
 
-template <int BOUND, int RNG, bool TRACE, bool MOMENT, int RECS>
+template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS>
 __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
                                           WaveScratch &L) {
     constexpr bool PERIODIC = BOUND != kBoundClamp;
+    constexpr bool TRACE = INSTR == kInstrTrace, STATS = INSTR == kInstrStats;
+    WalkCounters C;
+    if (STATS)
+        for (uint32_t k = 0; k < kWalkStats; ++k) C.v[k] = 0;
+    uint32_t shortcuts = 0;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t i = block * blockDim.x + threadIdx.x;
     // lanes past the last spin stay in the loop (done) for the wave operations
@@ -958,6 +996,10 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     uint32_t slice = 0;  // cells of the segment's box searched so far
     while (__ballot(it < iters) != 0ull) {
         bool complete = false;
+        if (STATS) {
+            C.v[kStatTrips] += 1;
+            C.v[kStatLiveLaneTrips] += popc(__ballot(it < iters));
+        }
         if (it < iters && !pending) {
             float n0, n1, n2;
             if (RNG == kRngInject) {
@@ -988,13 +1030,15 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
                     pending = false;
                     refresh = false;
                     complete = true;
+                    if (STATS) ++shortcuts;
                 }
             }
         }
         unsigned long long key;
         float4 hp;
         const bool searched =
-            wave_search<RECS>(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, slice, key, hp);
+            wave_search<RECS, STATS>(P, L, lane, pending, ox, oy, oz, px, py, pz, excluded, slice,
+                                     key, hp, C);
         if (pending && searched) {
             if (key == ~0ull) {
                 pending = false;
@@ -1107,16 +1151,25 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         }
         if (P.hits) P.hits[i] += nhit;
     }
+    if (STATS) {
+        C.v[kStatSteps] = popc(__ballot(live)) * iters;
+        C.v[kStatHits] = __builtin_amdgcn_readlane(wave_incl_sum(nhit), 63);
+        C.v[kStatShortcut] = __builtin_amdgcn_readlane(wave_incl_sum(shortcuts), 63);
+        C.v[kStatWaves] = 1;
+        if (lane == 0)
+            for (uint32_t k = 0; k < kWalkStats; ++k)
+                if (C.v[k]) atomicAdd(P.stats + k, (unsigned long long)C.v[k]);
+    }
 }
 
-template <int BOUND, int RNG, bool TRACE, bool MOMENT, int RECS>
+template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS>
 // One wave per workgroup: every wave runs until its own slowest lane is done,
 // and a finished wave frees its slot for the next block at once (with four
 // waves per workgroup the slot waited for the slowest of the four).
 __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
     StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[1];
-    walk_body<BOUND, RNG, TRACE, MOMENT, RECS>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
+    walk_body<BOUND, RNG, INSTR, MOMENT, RECS>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
 }
 
 // Batched walk for the server: the blocks of one launch are shared out over
@@ -1135,7 +1188,7 @@ __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu
     for (uint32_t st = kMaxJobs / 2; st; st >>= 1)
         if (j + st < T->n_jobs && T->block_start[j + st] <= b) j += st;
     const WalkJob &J = T->job[j];
-    walk_body<BOUND, kRngPhilox, false, true, RECS>(J.p, b - T->block_start[j], J.iters,
+    walk_body<BOUND, kRngPhilox, kInstrNone, true, RECS>(J.p, b - T->block_start[j], J.iters,
                                                     scratch[0]);
 }
 
@@ -1156,54 +1209,66 @@ hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool 
     return hipGetLastError();
 }
 
-template <int MODE, int BOUND, int RNG, bool TRACE, bool MOMENT>
+template <int MODE, int BOUND, int RNG, int INSTR, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     if constexpr (MODE == 0) {
-        hipLaunchKernelGGL((k_step<0, BOUND != kBoundClamp, RNG, TRACE, MOMENT>),
+        hipLaunchKernelGGL((k_step<0, BOUND != kBoundClamp, RNG, INSTR == kInstrTrace, MOMENT>),
                            dim3((p.n + 63) / 64), dim3(64), 0, s, p, iters);
     } else if (p.cell_mt) {
-        hipLaunchKernelGGL((k_walk<BOUND, RNG, TRACE, MOMENT, kRecsCellMajor>),
+        hipLaunchKernelGGL((k_walk<BOUND, RNG, INSTR, MOMENT, kRecsCellMajor>),
                            dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
                            dim3(kWalkBlockSpins), 0, s, p, iters);
     } else {
-        hipLaunchKernelGGL((k_walk<BOUND, RNG, TRACE, MOMENT, kRecsIndirect>),
+        hipLaunchKernelGGL((k_walk<BOUND, RNG, INSTR, MOMENT, kRecsIndirect>),
                            dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
                            dim3(kWalkBlockSpins), 0, s, p, iters);
     }
 }
 
+template <int MODE, int BOUND, int RNG, int INSTR>
+static void launch_m(const StepParams &p, uint32_t iters, bool moment, hipStream_t s) {
+    if (moment) launch_one<MODE, BOUND, RNG, INSTR, true>(p, iters, s);
+    else launch_one<MODE, BOUND, RNG, INSTR, false>(p, iters, s);
+}
+
+// The counting build of the walk exists for the physical mode with Philox
+// or injected draws (the runs whose work it measures, and the injected runs
+// that pin its counts to the oracle's).
 template <int MODE, int BOUND, int RNG>
-static hipError_t launch3(const StepParams &p, uint32_t iters, bool trace, bool moment,
+static hipError_t launch3(const StepParams &p, uint32_t iters, int instr, bool moment,
                           hipStream_t s) {
-    if (trace) {
-        if (moment) launch_one<MODE, BOUND, RNG, true, true>(p, iters, s);
-        else launch_one<MODE, BOUND, RNG, true, false>(p, iters, s);
+    if (instr == kInstrTrace) {
+        launch_m<MODE, BOUND, RNG, kInstrTrace>(p, iters, moment, s);
+    } else if (instr == kInstrStats) {
+        if constexpr (MODE == 1 && RNG != kRngMwc)
+            launch_m<MODE, BOUND, RNG, kInstrStats>(p, iters, moment, s);
+        else
+            return hipErrorInvalidValue;
     } else {
-        if (moment) launch_one<MODE, BOUND, RNG, false, true>(p, iters, s);
-        else launch_one<MODE, BOUND, RNG, false, false>(p, iters, s);
+        launch_m<MODE, BOUND, RNG, kInstrNone>(p, iters, moment, s);
     }
     return hipGetLastError();
 }
 
 template <int MODE, int BOUND>
-static hipError_t launch2(const StepParams &p, uint32_t iters, int rng, bool trace, bool moment,
+static hipError_t launch2(const StepParams &p, uint32_t iters, int rng, int instr, bool moment,
                           hipStream_t s) {
     switch (rng) {
-        case kRngInject: return launch3<MODE, BOUND, kRngInject>(p, iters, trace, moment, s);
-        case kRngMwc: return launch3<MODE, BOUND, kRngMwc>(p, iters, trace, moment, s);
-        default: return launch3<MODE, BOUND, kRngPhilox>(p, iters, trace, moment, s);
+        case kRngInject: return launch3<MODE, BOUND, kRngInject>(p, iters, instr, moment, s);
+        case kRngMwc: return launch3<MODE, BOUND, kRngMwc>(p, iters, instr, moment, s);
+        default: return launch3<MODE, BOUND, kRngPhilox>(p, iters, instr, moment, s);
     }
 }
 
 hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int periodic, int rng,
-                       bool trace, bool moment, hipStream_t s) {
+                       int instr, bool moment, hipStream_t s) {
     if (p.n == 0 || iters == 0) return hipSuccess;
-    if (mode == 0) return launch2<0, kBoundClamp>(p, iters, rng, trace, moment, s);
-    if (!periodic) return launch2<1, kBoundClamp>(p, iters, rng, trace, moment, s);
+    if (mode == 0) return launch2<0, kBoundClamp>(p, iters, rng, instr, moment, s);
+    if (!periodic) return launch2<1, kBoundClamp>(p, iters, rng, instr, moment, s);
     bool cube = true;
     for (int k = 0; k < 3; ++k) cube &= p.wrap_lo[k] == -1.0f && p.wrap_hi[k] == 1.0f;
-    if (cube) return launch2<1, kBoundCube>(p, iters, rng, trace, moment, s);
-    return launch2<1, kBoundBox>(p, iters, rng, trace, moment, s);
+    if (cube) return launch2<1, kBoundCube>(p, iters, rng, instr, moment, s);
+    return launch2<1, kBoundBox>(p, iters, rng, instr, moment, s);
 }
 
 // ------------------------------------------------------------------------ //

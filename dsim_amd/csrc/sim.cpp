@@ -51,6 +51,8 @@ struct dsim_sim {
     DevBuf<float> grads;
     DevBuf<float4> inject;
     DevBuf<uint32_t> trace;
+    DevBuf<unsigned long long> stats;  // walk work counters while counting is on
+    bool counting = false;
     DevBuf<uint32_t> keys_a, keys_b, vals_a, vals_b;
     DevBuf<uint8_t> sort_tmp;
     uint64_t steps_since_sort = ~0ull;
@@ -72,6 +74,7 @@ struct dsim_sim {
                         &vals_b})
             b->release();
         partial.release(); out.release(); grads.release(); inject.release(); sort_tmp.release();
+        stats.release();
         if (stream) (void)hipStreamDestroy(stream);
         if (sub) sub->refs--;
     }
@@ -413,6 +416,13 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
             if (int rc = s->trace.alloc((size_t)iters * n * 5)) return rc;
         }
     }
+    int instr = trace ? dsim::kInstrTrace : dsim::kInstrNone;
+    if (s->counting && !trace) {
+        if (s->prm.mode != DSIM_MODE_PHYSICAL || rng == dsim::kRngMwc)
+            return fail(DSIM_ERR_ARG,
+                        "walk counters need the physical mode with Philox or injected draws");
+        instr = dsim::kInstrStats;
+    }
     uint32_t done = 0;
     const uint32_t chunk = s->sorting() ? std::min(s->prm.steps_per_launch, s->prm.sort_interval)
                                         : s->prm.steps_per_launch;
@@ -429,6 +439,7 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         P.step0 = s->step + done;
         P.inject = n3u ? s->inject.p + (size_t)done * n : nullptr;
         P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
+        P.stats = instr == dsim::kInstrStats ? s->stats.p : nullptr;
         // each step launch is bracketed by its own events: the timing covers
         // the step kernel only, not the reorders between launches
         if (s->ev.size() >= kMaxEventPairs)
@@ -444,7 +455,7 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         }
         s->ev.emplace_back(e0, e1);  // owned by the handle from here on
         e = dsim::launch_step(P, k, (int)s->prm.mode, s->prm.boundary == DSIM_BOUNDARY_PERIODIC,
-                              rng, trace != nullptr, moment, s->stream);
+                              rng, instr, moment, s->stream);
         if (e != hipSuccess) return fail(DSIM_ERR_HIP, "step launch: %s", hipGetErrorString(e));
         HIP_TRY(hipEventRecord(e1, s->stream));
         done += k;
@@ -692,6 +703,32 @@ extern "C" int dsim_step_timing_reset(dsim_sim *s) {
     if (int rc = dsim_step_timing(s, &t, nullptr)) return rc;
     s->ms_done = 0.0;
     s->launches = 0;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_walk_stats_enable(dsim_sim *s, int enable) {
+    if (int rc = check_sim(s)) return rc;
+    if (enable) {
+        if (!s->stats.p)
+            if (int rc = s->stats.alloc(dsim::kWalkStats)) return rc;
+        HIP_TRY(hipMemsetAsync(s->stats.p, 0, dsim::kWalkStats * sizeof(unsigned long long),
+                               s->stream));
+    }
+    s->counting = enable != 0;
+    return DSIM_OK;
+}
+
+extern "C" int dsim_walk_stats_read(dsim_sim *s, uint64_t counts[DSIM_WALK_STATS]) {
+    if (int rc = check_sim(s)) return rc;
+    if (!counts) return fail(DSIM_ERR_ARG, "NULL counts");
+    static_assert(DSIM_WALK_STATS == dsim::kWalkStats, "counter count");
+    if (!s->stats.p) {
+        for (int k = 0; k < DSIM_WALK_STATS; ++k) counts[k] = 0;
+        return DSIM_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(counts, s->stats.p, dsim::kWalkStats * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return DSIM_OK;
 }
 

@@ -21,6 +21,10 @@ ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "OVERFLOW", -6: "IO",
 NO_TRIANGLE = 0xFFFFFFFF
 NO_FIBER = 65535
 GRID_REFERENCE, GRID_AABB, GRID_TRIBOX = 0, 1, 2
+# DSIM_STAT_* of include/dsim.h, in index order
+WALK_STATS = ("steps", "trips", "search_lanes", "cells", "plane_tests", "plane_passes",
+              "tri_tests", "tri_passes", "fp64_tests", "hits", "shortcuts", "deferred",
+              "waves", "live_lanes")
 MODE_REFERENCE, MODE_PHYSICAL = 0, 1
 BOUNDARY_CLAMP, BOUNDARY_PERIODIC = 0, 1
 RNG_MWC, RNG_PHILOX = 1, 2
@@ -125,6 +129,8 @@ def lib():
             "dsim_sync": [_vp],
             "dsim_step_timing": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
             "dsim_step_timing_reset": [_vp],
+            "dsim_walk_stats_enable": [_vp, C.c_int],
+            "dsim_walk_stats_read": [_vp, C.POINTER(C.c_uint64)],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -398,6 +404,18 @@ class Simulation:
 
     def step_timing_reset(self):
         check(lib().dsim_step_timing_reset(self.h))
+
+    def walk_stats(self, enable=None):
+        """Work counters of the physical-mode walk (a dict keyed by the
+        WALK_STATS names).  walk_stats(True) zeroes them and switches the
+        counting build on; walk_stats(False) switches it off; walk_stats()
+        reads them."""
+        if enable is not None:
+            check(lib().dsim_walk_stats_enable(self.h, int(bool(enable))))
+            return None
+        a = (C.c_uint64 * 16)()
+        check(lib().dsim_walk_stats_read(self.h, a))
+        return {k: int(a[i]) for i, k in enumerate(WALK_STATS)}
 
 
 # --------------------------------------------------------------------------- #

@@ -278,6 +278,32 @@ int dsim_sync(dsim_sim *sim);
 int dsim_step_timing(dsim_sim *sim, double *ms, uint64_t *launches);
 int dsim_step_timing_reset(dsim_sim *sim);
 
+/* No reference counterpart: work counters of the physical-mode walk.
+ * While counting is on, the integrate calls run a counting build of the walk
+ * (same results, slower); the counts add up over calls until counting is
+ * switched on again (which zeroes them).  Counting on a handle in the
+ * reference mode or with MWC draws fails with DSIM_ERR_ARG at the integrate
+ * call; Philox and injected draws are counted. */
+#define DSIM_WALK_STATS 16
+enum {
+    DSIM_STAT_STEPS = 0,         /* spin-steps completed */
+    DSIM_STAT_TRIPS = 1,         /* trips round a wave's loop (one search round each) */
+    DSIM_STAT_SEARCH_LANES = 2,  /* lanes searching, summed over trips */
+    DSIM_STAT_CELLS = 3,         /* grid cells listed by searching lanes */
+    DSIM_STAT_PLANE_TESTS = 4,   /* (segment, grid entry) plane tests */
+    DSIM_STAT_PLANE_PASSES = 5,  /* wave passes of the plane test */
+    DSIM_STAT_TRI_TESTS = 6,     /* Moller-Trumbore tests (plane crossings) */
+    DSIM_STAT_TRI_PASSES = 7,    /* wave passes of the triangle test */
+    DSIM_STAT_FP64_TESTS = 8,    /* near-grazing FP64 re-tests */
+    DSIM_STAT_HITS = 9,          /* membrane hits */
+    DSIM_STAT_SHORTCUTS = 10,    /* steps completed by the clearance shortcut */
+    DSIM_STAT_DEFERRED = 11,     /* lanes whose cell list waited a round, summed over trips */
+    DSIM_STAT_WAVES = 12,        /* waves run */
+    DSIM_STAT_LIVE_LANES = 13    /* lanes with steps left, summed over trips */
+};
+int dsim_walk_stats_enable(dsim_sim *sim, int enable);
+int dsim_walk_stats_read(dsim_sim *sim, uint64_t counts[DSIM_WALK_STATS]);
+
 #ifdef __cplusplus
 }
 #endif
