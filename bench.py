@@ -370,12 +370,13 @@ def substrate_prep(env):
             "identical": same}
 
 
-def waveform_bench(n_spins, n_wave, sample, chunk=256):
+def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=1024):
     """Waveform-phase path: n_spins spins on the small substrate, n_wave random
     piecewise-constant waveforms (|G| <= 80 mT/m, a new vector every 10 us) over
-    12,000 steps in chunks of `chunk` steps; the phase GEMM against the HBM peak;
-    per-waveform error against FP64 on a `sample`-spin run of the same
-    substrate and waveforms."""
+    12,000 steps, Kahan-folded every `chunk` steps (GEMM K = 3 * chunk), the
+    trajectory walked `buffer` steps per GEMM launch; the phase GEMM against
+    the dense BF16 MFMA peak; per-waveform error against FP64 on a
+    `sample`-spin run of the same substrate and waveforms."""
     from dsim_amd import phase as PH
 
     cfg = S.SMALL
@@ -399,7 +400,7 @@ def waveform_bench(n_spins, n_wave, sample, chunk=256):
         return sim
 
     sim = prepared(n_spins, 3)
-    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk)
+    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk, buffer_steps=buffer)
     t0 = time.perf_counter()
     wf.run(steps)
     sig = wf.signal()
@@ -408,8 +409,8 @@ def waveform_bench(n_spins, n_wave, sample, chunk=256):
     wf.close()
     sim.close()
     flops = 3 * 2.0 * n_spins * n_wave * 3 * steps   # hi*hi + hi*lo + lo*hi
-    n_chunks = -(-steps // chunk)
-    gemm_bytes = n_chunks * (16.0 * n_spins * n_wave) + 4.0 * n_spins * 3 * steps
+    n_buffers = -(-steps // buffer)
+    gemm_bytes = n_buffers * (16.0 * n_spins * n_wave) + 4.0 * n_spins * 3 * steps
     log(f"waveforms: {n_spins} spins x {steps} steps x {n_wave} waveforms in {wall:.2f} s "
         f"(walk {walk_ms / 1e3:.2f} s, phase GEMM {gemm_ms / 1e3:.2f} s = "
         f"{flops / (gemm_ms / 1e3) / 1e12:.0f} TFLOP/s); b=0-free signal range "
@@ -417,7 +418,7 @@ def waveform_bench(n_spins, n_wave, sample, chunk=256):
     # FP64 check on a sample: the same product against a float64 GEMM of the
     # recorded trajectory (numpy, checker only)
     sim = prepared(sample, 4)
-    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk)
+    wf = PH.Waveforms(sim, grads, pc, chunk_steps=chunk, buffer_steps=buffer)
     traj = wf.run(steps, trajectory=True)
     phi, _ = wf.phase()
     got = wf.signal()
@@ -443,19 +444,22 @@ def waveform_bench(n_spins, n_wave, sample, chunk=256):
         f"{np.median(dsig):.2e})")
     sub.close()
     return {"spins": n_spins, "waveforms": n_wave, "steps": steps, "chunk_steps": chunk,
-            "gemm_k": 3 * chunk, "substrate": cfg.name, "seconds": wall,
+            "gemm_k": 3 * chunk, "buffer_steps": buffer, "substrate": cfg.name, "seconds": wall,
             "spin_steps_per_s": n_spins * steps / wall, "walk_s": walk_ms / 1e3,
             "phase_gemm_s": gemm_ms / 1e3,
-            # algorithmic bytes: the read-modify-write of the (phase,
-            # compensation) pairs, 8 B per spin and waveform each way, once per
-            # chunk, plus one read of the chunk's trajectory (4 B per spin and k)
-            "phase_gemm_roofline": {"bound": "hbm", "achieved": gemm_bytes / (gemm_ms / 1e3) / 1e9,
-                                    "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                                    "frac": gemm_bytes / (gemm_ms / 1e3) / PEAK_HBM,
-                                    "traffic": committed_gemm_traffic(n_spins, n_wave, chunk),
-                                    "traffic_unit": "bytes per full chunk",
-                                    "mfma_tflops": flops / (gemm_ms / 1e3) / 1e12,
-                                    "mfma_frac_of_bf16_peak": flops / (gemm_ms / 1e3) / PEAK_BF16},
+            # algorithmic flops: the three bf16 products (hi*hi, hi*lo, lo*hi)
+            # of the spins x (3 * steps) x waveforms GEMM; bytes: the
+            # read-modify-write of the (phase, compensation) pairs, 8 B per
+            # spin and waveform each way, once per trajectory buffer, plus one
+            # read of the trajectory (4 B per spin and k)
+            "phase_gemm_roofline": {"bound": "mfma", "achieved": flops / (gemm_ms / 1e3) / 1e12,
+                                    "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                                    "frac": flops / (gemm_ms / 1e3) / PEAK_BF16,
+                                    "hbm_gbs": gemm_bytes / (gemm_ms / 1e3) / 1e9,
+                                    "hbm_frac": gemm_bytes / (gemm_ms / 1e3) / PEAK_HBM,
+                                    "traffic": committed_gemm_traffic(n_spins, n_wave, chunk,
+                                                                      buffer),
+                                    "traffic_unit": "bytes per full trajectory buffer"},
             "fp64_check": {"spins": sample, "phase_err_max_rad": float(dphi.max()),
                            "phase_err_median_rad": float(np.median(dphi)),
                            "signal_err_max": float(dsig.max()),
@@ -485,18 +489,18 @@ def committed_traffic(cell_um, sort_interval, n):
 GEMM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_phase_gemm_small-16axon-4e6x1024_hbm_traffic.md")
 
 
-def committed_gemm_traffic(n_spins, n_wave, chunk):
-    """HBM bytes per full-chunk launch of k_phase_gemm from the committed
+def committed_gemm_traffic(n_spins, n_wave, chunk, buffer):
+    """HBM bytes per full-buffer launch of k_phase_gemm from the committed
     FETCH_SIZE / WRITE_SIZE summary for the matching sizes; else None."""
     try:
         lines = open(GEMM_TRAFFIC_FILE).read().splitlines()
     except OSError:
         return None
     for ln in lines:
-        if ln.startswith("traffic_bytes_per_chunk"):
+        if ln.startswith("traffic_bytes_per_buffer"):
             f = dict(kv.split("=") for kv in ln.split()[1:])
             if (int(f["spins"]) == n_spins and int(f["waveforms"]) == n_wave
-                    and int(f["chunk"]) == chunk):
+                    and int(f["chunk"]) == chunk and int(f["buffer"]) == buffer):
                 return int(f["bytes"])
     return None
 
@@ -543,8 +547,10 @@ def main():
     ap.add_argument("--waveforms", type=int, default=1024)
     ap.add_argument("--reference-spins", type=int, default=1_000_000,
                     help="spins of the reference-semantics run (0 skips it)")
-    ap.add_argument("--chunk-steps", type=int, default=256,
-                    help="steps per phase-GEMM chunk of the waveform run (multiple of 32)")
+    ap.add_argument("--chunk-steps", type=int, default=64,
+                    help="Kahan interval of the waveform phases in steps (multiple of 32)")
+    ap.add_argument("--buffer-steps", type=int, default=1024,
+                    help="steps walked per phase-GEMM launch (multiple of --chunk-steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -664,7 +670,8 @@ def main():
             e["sub"].close()
     waves = None
     if args.waveform_spins:
-        waves = waveform_bench(args.waveform_spins, args.waveforms, 8192, args.chunk_steps)
+        waves = waveform_bench(args.waveform_spins, args.waveforms, 8192, args.chunk_steps,
+                               args.buffer_steps)
     refmode = None
     if args.reference_spins:
         refmode = reference_mode_bench(args.reference_spins, 1000, 5.0, hcpu["threads_used"],

@@ -166,9 +166,12 @@ constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout k
 constexpr uint32_t kPhaseTileM = 128;  // spins per workgroup tile
 constexpr uint32_t kPhaseTileN = 128;  // waveforms per workgroup tile
 constexpr uint32_t kPhaseSliceK = 32;  // k per LDS stage
-hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
-                             const uint16_t *g_hi, const uint16_t *g_lo, uint32_t k_total,
-                             uint32_t k0, uint32_t n_wave, float2 *phi, hipStream_t s);
+// k_len trajectory rows in Kahan folds of k_fold rows (both multiples of
+// kPhaseSliceK; the last fold may be shorter).
+hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_len,
+                             uint32_t k_fold, const uint16_t *g_hi, const uint16_t *g_lo,
+                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float2 *phi,
+                             hipStream_t s);
 // Signal of every waveform: per spin block and waveform the FP64 sums of
 // m cos(phase), m sin(phase) and counts for the total and compartments 0-3,
 // phase = phi[s][w] + gsum[w] . x0[s] (in FP64); then a fixed-order final

@@ -1283,10 +1283,12 @@ hipError_t launch_step(const StepParams &p, uint32_t iters, int mode, int period
 // Kahan compensation.
 //
 // Workgroup: 256 threads, tile of 128 spins x 128 waveforms, four waves of
-// 64 x 64 (2 x 2 MFMA tiles of 32 x 32), several workgroups per CU.  K advances in slices of 32: the
-// trajectory slice (fp32, spins contiguous) is split and transposed into LDS
-// as [spin][k], the waveform slice is copied as [waveform][k], so every MFMA
-// operand (8 consecutive k of one row) is one 16-byte LDS read.
+// 64 x 64 (2 x 2 MFMA tiles of 32 x 32), whose (phase, compensation) pairs
+// stay in registers for the whole launch (one wave per SIMD).  K advances in
+// slices of 32 through two LDS stages: the trajectory slice (fp32, spins
+// contiguous) is split and transposed into LDS as [spin][k], the waveform
+// slice is copied as [waveform][k], so every MFMA operand (8 consecutive k of
+// one row) is one 16-byte LDS read.
 // v_mfma_f32_32x32x16_bf16 lane layout (checked on the device): A row l%32,
 // B column l%32, k = 8*(l/32) + v for v < 8; result register r holds row
 // 8*(r/4) + 4*(l/32) + r%4, column l%32.
@@ -1303,20 +1305,77 @@ __device__ __forceinline__ uint16_t bf16_bits_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Register stage of one K slice of the phase GEMM (see k_phase_gemm).
+struct PhaseStage {
+    float tv[16];
+    uint4 wh0, wh1, wl0, wl1;
+};
+__device__ __forceinline__ void phase_gload(PhaseStage &S, const float *__restrict__ traj,
+                                            uint32_t n_pad, uint32_t spin, uint32_t akg,
+                                            uint32_t ks, const uint16_t *__restrict__ g_hi,
+                                            const uint16_t *__restrict__ g_lo, uint32_t n0,
+                                            uint32_t k_total, uint32_t k0, uint32_t t) {
+    const float *src = traj + (size_t)(ks + akg) * n_pad + spin;
+#pragma unroll
+    for (uint32_t q = 0; q < 16; ++q) S.tv[q] = src[(size_t)q * n_pad];
+    // 16-byte piece c: row c/4, quarter c%4, for c = t and t + 256
+    const size_t g0 = (size_t)(n0 + (t >> 2)) * k_total + k0 + ks + (t & 3u) * 8u;
+    const size_t g1 = g0 + (size_t)64 * k_total;
+    S.wh0 = *reinterpret_cast<const uint4 *>(g_hi + g0);
+    S.wl0 = *reinterpret_cast<const uint4 *>(g_lo + g0);
+    S.wh1 = *reinterpret_cast<const uint4 *>(g_hi + g1);
+    S.wl1 = *reinterpret_cast<const uint4 *>(g_lo + g1);
+}
+__device__ __forceinline__ uint4 pack_bf16x8(const uint16_t *h) {
+    uint4 v;
+    v.x = h[0] | ((uint32_t)h[1] << 16);
+    v.y = h[2] | ((uint32_t)h[3] << 16);
+    v.z = h[4] | ((uint32_t)h[5] << 16);
+    v.w = h[6] | ((uint32_t)h[7] << 16);
+    return v;
+}
+// Trajectory split into bf16 hi + lo as [spin][k] rows (two 16-byte rows
+// each, distinct banks for 8 consecutive spins at the 80-byte pitch);
+// waveform rows as loaded.
+__device__ __forceinline__ void phase_lstore(const PhaseStage &S, uint16_t *a_hi, uint16_t *a_lo,
+                                             uint16_t *b_hi, uint16_t *b_lo, uint32_t am,
+                                             uint32_t akg, uint32_t t) {
+    uint16_t h[16], l[16];
+#pragma unroll
+    for (uint32_t q = 0; q < 16; ++q) {
+        h[q] = bf16_bits_rne(S.tv[q]);
+        l[q] = bf16_bits_rne(S.tv[q] - bf16_to_float(h[q]));
+    }
+    *reinterpret_cast<uint4 *>(&a_hi[am * kLdsPitch + akg]) = pack_bf16x8(h);
+    *reinterpret_cast<uint4 *>(&a_hi[am * kLdsPitch + akg + 8]) = pack_bf16x8(h + 8);
+    *reinterpret_cast<uint4 *>(&a_lo[am * kLdsPitch + akg]) = pack_bf16x8(l);
+    *reinterpret_cast<uint4 *>(&a_lo[am * kLdsPitch + akg + 8]) = pack_bf16x8(l + 8);
+    const uint32_t r0 = t >> 2, part = (t & 3u) * 8u;
+    *reinterpret_cast<uint4 *>(&b_hi[r0 * kLdsPitch + part]) = S.wh0;
+    *reinterpret_cast<uint4 *>(&b_lo[r0 * kLdsPitch + part]) = S.wl0;
+    *reinterpret_cast<uint4 *>(&b_hi[(r0 + 64) * kLdsPitch + part]) = S.wh1;
+    *reinterpret_cast<uint4 *>(&b_lo[(r0 + 64) * kLdsPitch + part]) = S.wl1;
+}
+
+// One launch covers k_len trajectory rows (the steps of one trajectory
+// buffer) in folds of k_fold rows: each fold's products are summed in the
+// MFMA accumulators from zero and then added, with Kahan compensation, to the
+// (phase, compensation) pairs, which stay in registers from the first fold
+// to the last and touch memory once per launch.  The K loop stages slices of
+// 32 through LDS, the next slice's global loads in flight while the current
+// slice's MFMAs run.
 __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ traj, uint32_t n,
-                                                    uint32_t n_pad, uint32_t k_chunk,
+                                                    uint32_t n_pad, uint32_t k_len,
+                                                    uint32_t k_fold,
                                                     const uint16_t *__restrict__ g_hi,
                                                     const uint16_t *__restrict__ g_lo,
                                                     uint32_t k_total, uint32_t k0,
                                                     uint32_t n_wave, float2 *__restrict__ phi) {
-    // K loop: bf16 hi/lo operand slices; epilogue: the same memory holds half
-    // of the FP32 result tile (64 spins x 128 waveforms)
+    // two LDS stages: the MFMAs of slice s read one while slice s + 1 is split
+    // and stored into the other, one barrier per slice
     constexpr uint32_t kOperandHalves = kPhaseTileM * kLdsPitch;
-    __shared__ __align__(16) uint16_t lds[2 * kOperandHalves + 2 * kPhaseTileN * kLdsPitch];
-    static_assert(sizeof(lds) >= 64 * kPhaseTileN * sizeof(float), "epilogue stage must fit");
-    uint16_t *a_hi = lds, *a_lo = lds + kOperandHalves;
-    uint16_t *b_hi = lds + 2 * kOperandHalves, *b_lo = b_hi + kPhaseTileN * kLdsPitch;
-    float *stage = reinterpret_cast<float *>(lds);
+    constexpr uint32_t kStage = 2 * kOperandHalves + 2 * kPhaseTileN * kLdsPitch;
+    __shared__ __align__(16) uint16_t lds[2 * kStage];
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     // Workgroups go to the 8 XCDs in turn; the numbering below hands all
     // waveform tiles of one spin tile to one XCD back to back, so the
@@ -1328,57 +1387,50 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
     const uint32_t n0 = (r % n_tiles) * kPhaseTileN;  // waveform tile
     const uint32_t m0 = tm * kPhaseTileM;             // spin tile
     const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 64u;  // wave tile in the WG tile
+    // v_mfma_f32_32x32x16_bf16 result register q of lane l: row
+    // 8*(q/4) + 4*(l/32) + q%4, column l%32 of the 32 x 32 tile
+    auto spin_of = [&](int i, int q) -> uint32_t {
+        return m0 + wm + 32u * i + 8u * (q / 4) + 4u * (lane >> 5) + (q % 4);
+    };
+    auto wave_of = [&](int j) -> uint32_t { return n0 + wn + 32u * j + (lane & 31u); };
+    float ps[2][2][16], pk[2][2][16];  // phase and its Kahan compensation
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t sp = spin_of(i, q);
+                const float2 v = sp < n ? phi[(size_t)sp * n_wave + wave_of(j)] : make_float2(0.f, 0.f);
+                ps[i][j][q] = v.x;
+                pk[i][j][q] = v.y;
+            }
+    // global -> register stage of a slice: the thread's spin (t % 128) and 16
+    // consecutive k of the trajectory; two 16-byte pieces of hi and of lo
+    // waveform rows
+    const uint32_t am = t & 127u, akg = (t >> 7) * 16u;
+    PhaseStage stg;
+    const uint32_t n_slices = k_len / kPhaseSliceK, fold_slices = k_fold / kPhaseSliceK;
+    phase_gload(stg, traj, n_pad, m0 + am, akg, 0, g_hi, g_lo, n0, k_total, k0, t);
+    {
+        uint16_t *st = lds;
+        phase_lstore(stg, st, st + kOperandHalves, st + 2 * kOperandHalves,
+                     st + 2 * kOperandHalves + kPhaseTileN * kLdsPitch, am, akg, t);
+    }
+    phase_gload(stg, traj, n_pad, m0 + am, akg, min(1u, n_slices - 1) * kPhaseSliceK, g_hi, g_lo,
+                n0, k_total, k0, t);
+    __syncthreads();
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-
-    for (uint32_t ks = 0; ks < k_chunk; ks += kPhaseSliceK) {
-        __syncthreads();
-        // trajectory slice: 32 rows x 128 spins (fp32) -> hi/lo [spin][k].
-        // Thread: one spin, 16 consecutive k (loads coalesced over the spins);
-        // its 16 hi and 16 lo values go to LDS as two 16-byte rows each, which
-        // at an 80-byte pitch land on distinct banks for 8 consecutive spins.
-        {
-            const uint32_t m = t & 127u, kg = (t >> 7) * 16u;
-            const float *src = traj + (size_t)(ks + kg) * n_pad + m0 + m;
-            uint16_t h[16], l[16];
-#pragma unroll
-            for (uint32_t q = 0; q < 16; ++q) {
-                const float x = src[(size_t)q * n_pad];
-                h[q] = bf16_bits_rne(x);
-                l[q] = bf16_bits_rne(x - bf16_to_float(h[q]));
-            }
-#pragma unroll
-            for (uint32_t q = 0; q < 2; ++q) {
-                uint4 vh, vl;
-                vh.x = h[8 * q] | ((uint32_t)h[8 * q + 1] << 16);
-                vh.y = h[8 * q + 2] | ((uint32_t)h[8 * q + 3] << 16);
-                vh.z = h[8 * q + 4] | ((uint32_t)h[8 * q + 5] << 16);
-                vh.w = h[8 * q + 6] | ((uint32_t)h[8 * q + 7] << 16);
-                vl.x = l[8 * q] | ((uint32_t)l[8 * q + 1] << 16);
-                vl.y = l[8 * q + 2] | ((uint32_t)l[8 * q + 3] << 16);
-                vl.z = l[8 * q + 4] | ((uint32_t)l[8 * q + 5] << 16);
-                vl.w = l[8 * q + 6] | ((uint32_t)l[8 * q + 7] << 16);
-                *reinterpret_cast<uint4 *>(&a_hi[m * kLdsPitch + kg + 8 * q]) = vh;
-                *reinterpret_cast<uint4 *>(&a_lo[m * kLdsPitch + kg + 8 * q]) = vl;
-            }
-        }
-        // waveform slice: 128 rows x 32 k (bf16, k contiguous), 64 bytes a row
-#pragma unroll
-        for (uint32_t q = 0; q < 2; ++q) {
-            const uint32_t c = t + 256u * q;  // 16-byte piece: row c/4, quarter c%4
-            const uint32_t row = c >> 2, part = (c & 3u) * 8u;
-            const size_t src = (size_t)(n0 + row) * k_total + k0 + ks + part;
-            *reinterpret_cast<uint4 *>(&b_hi[row * kLdsPitch + part]) =
-                *reinterpret_cast<const uint4 *>(g_hi + src);
-            *reinterpret_cast<uint4 *>(&b_lo[row * kLdsPitch + part]) =
-                *reinterpret_cast<const uint4 *>(g_lo + src);
-        }
-        __syncthreads();
+            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+    for (uint32_t sl = 0; sl < n_slices; ++sl) {
+        const uint16_t *cur = lds + (sl & 1u) * kStage;
+        const uint16_t *a_hi = cur, *a_lo = cur + kOperandHalves;
+        const uint16_t *b_hi = cur + 2 * kOperandHalves, *b_lo = b_hi + kPhaseTileN * kLdsPitch;
 #pragma unroll
         for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
             const uint32_t kofs = kk + 8u * (lane >> 5);
@@ -1398,62 +1450,69 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
+                for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                }
-        }
-    }
-    // Kahan fold into (phase, compensation) pairs [spin][waveform], one half of
-    // the tile (64 spins) at a time through LDS: the two waves holding those
-    // spins stage their results, then every thread updates 16 pieces of 16
-    // bytes (two waveforms each), a wave covering one 1 KB row per access,
-    // with all 16 loads in flight before any arithmetic.
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
-        __syncthreads();
-        if (wm == 64u * h) {
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+        // the next slice into the other stage while these MFMAs run, and the
+        // one after it into registers (unconditionally, so that the compiler
+        // can interleave the split with the MFMAs: after the last slice the
+        // other stage is written and never read, and the loads stay on the
+        // last slice)
+        {
+            uint16_t *nxt = lds + ((sl + 1) & 1u) * kStage;
+            phase_lstore(stg, nxt, nxt + kOperandHalves, nxt + 2 * kOperandHalves,
+                         nxt + 2 * kOperandHalves + kPhaseTileN * kLdsPitch, am, akg, t);
+            phase_gload(stg, traj, n_pad, m0 + am, akg,
+                        min(sl + 2, n_slices - 1) * kPhaseSliceK, g_hi, g_lo, n0, k_total, k0, t);
+        }
+        if ((sl + 1) % fold_slices == 0 || sl + 1 == n_slices) {
+            // Kahan fold of this chunk's sums
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const uint32_t row = 32u * i + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
-                        stage[row * kPhaseTileN + wn + 32u * j + (lane & 31u)] = acc[i][j][r];
+                    for (int q = 0; q < 16; ++q) {
+                        const float y = acc[i][j][q] - pk[i][j][q];
+                        const float s1 = ps[i][j][q] + y;
+                        pk[i][j][q] = (s1 - ps[i][j][q]) - y;
+                        ps[i][j][q] = s1;
+                        acc[i][j][q] = 0.0f;
                     }
         }
         __syncthreads();
-        const uint32_t c = t & 63u, r0 = t >> 6;
-        float4 v[16];
-#pragma unroll
-        for (uint32_t q = 0; q < 16; ++q) {
-            const uint32_t s = m0 + 64u * h + r0 + 4u * q;
-            v[q] = s < n ? *reinterpret_cast<const float4 *>(phi + (size_t)s * n_wave + n0 + 2u * c)
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < 16; ++q) {
-            const uint32_t rr = r0 + 4u * q, s = m0 + 64u * h + rr;
-            if (s >= n) continue;
-            const float2 d = *reinterpret_cast<const float2 *>(&stage[rr * kPhaseTileN + 2u * c]);
-            const float y0 = d.x - v[q].y, t0 = v[q].x + y0;
-            const float y1 = d.y - v[q].w, t1 = v[q].z + y1;
-            *reinterpret_cast<float4 *>(phi + (size_t)s * n_wave + n0 + 2u * c) =
-                make_float4(t0, (t0 - v[q].x) - y0, t1, (t1 - v[q].z) - y1);
-        }
     }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t sp = spin_of(i, q);
+                if (sp < n) phi[(size_t)sp * n_wave + wave_of(j)] = make_float2(ps[i][j][q], pk[i][j][q]);
+            }
 }
 
-hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_chunk,
-                             const uint16_t *g_hi, const uint16_t *g_lo, uint32_t k_total,
-                             uint32_t k0, uint32_t n_wave, float2 *phi, hipStream_t s) {
-    if (n == 0) return hipSuccess;
+hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_len,
+                             uint32_t k_fold, const uint16_t *g_hi, const uint16_t *g_lo,
+                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float2 *phi,
+                             hipStream_t s) {
+    if (n == 0 || k_len == 0) return hipSuccess;
+    if (k_fold == 0 || k_fold % kPhaseSliceK || k_len % kPhaseSliceK) return hipErrorInvalidValue;
     const uint32_t m_groups = (n_pad / kPhaseTileM + 7u) / 8u;
     const uint32_t grid = m_groups * 8u * (n_wave / kPhaseTileN);
-    hipLaunchKernelGGL(k_phase_gemm, dim3(grid), dim3(256), 0, s, traj, n, n_pad, k_chunk, g_hi,
-                       g_lo, k_total, k0, n_wave, phi);
+    hipLaunchKernelGGL(k_phase_gemm, dim3(grid), dim3(256), 0, s, traj, n, n_pad, k_len, k_fold,
+                       g_hi, g_lo, k_total, k0, n_wave, phi);
     return hipGetLastError();
 }
 

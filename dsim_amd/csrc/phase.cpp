@@ -22,9 +22,10 @@ using dsim_host::upload;
 struct dsim_waveforms {
     dsim_sim *sim = nullptr;
     uint32_t n = 0, n_pad = 0, n_wave = 0, n_steps = 0, chunk = 0, done = 0;
+    uint32_t buffer = 0;          // steps walked per trajectory buffer (a multiple of chunk)
     DevBuf<uint16_t> g_hi, g_lo;  // [n_wave][3*n_steps] bf16 bits, pc folded in
     DevBuf<double> gsum;          // [n_wave][3]: sum_t pc * G_w(t)
-    DevBuf<float> traj;           // [3*chunk][n_pad]
+    DevBuf<float> traj;           // [3*buffer][n_pad]
     DevBuf<float2> phi;           // [n][n_wave]: (phase, Kahan compensation)
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
@@ -67,6 +68,7 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     W->n_wave = n_wave;
     W->n_steps = n_steps;
     W->chunk = chunk;
+    W->buffer = chunk;
     const size_t K = (size_t)3 * n_steps;
     std::vector<uint16_t> hi(n_wave * K), lo(n_wave * K);
     std::vector<double> gs((size_t)n_wave * 3, 0.0);
@@ -99,6 +101,21 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     return DSIM_OK;
 }
 
+extern "C" int dsim_waveforms_set_buffer(dsim_waveforms *W, uint32_t buffer_steps) {
+    if (!W) return fail(DSIM_ERR_ARG, "NULL waveforms");
+    if (buffer_steps == 0 || buffer_steps % W->chunk)
+        return fail(DSIM_ERR_ARG, "buffer_steps must be a positive multiple of chunk_steps (%u)",
+                    W->chunk);
+    if (buffer_steps == W->buffer) return DSIM_OK;
+    HIP_TRY(hipStreamSynchronize(dsim_host::sim_stream(W->sim)));
+    W->traj.release();
+    if (int rc = W->traj.alloc((size_t)3 * buffer_steps * W->n_pad)) return rc;
+    // padded trajectory columns stay zero
+    HIP_TRY(hipMemset(W->traj.p, 0, W->traj.n * 4));
+    W->buffer = buffer_steps;
+    return DSIM_OK;
+}
+
 extern "C" int dsim_waveforms_destroy(dsim_waveforms *W) {
     if (!W) return DSIM_OK;
     (void)hipStreamSynchronize(dsim_host::sim_stream(W->sim));
@@ -116,11 +133,11 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     int rc = DSIM_OK;
     for (uint32_t c = 0; c < steps && rc == DSIM_OK;) {
-        const uint32_t len = std::min(W->chunk, steps - c);  // a multiple of 32
+        const uint32_t len = std::min(W->buffer, steps - c);  // a multiple of 32
         HIP_TRY(hipEventRecord(ev[0], st));
         if ((rc = dsim_host::sim_walk_traj(W->sim, len, W->traj.p, W->n_pad))) break;
         HIP_TRY(hipEventRecord(ev[1], st));
-        hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len,
+        hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len, 3 * W->chunk,
                                                W->g_hi.p, W->g_lo.p, 3 * W->n_steps,
                                                3 * W->done, W->n_wave, W->phi.p, st);
         if (e != hipSuccess) {

@@ -23,6 +23,7 @@ def _lib():
             "dsim_waveforms_create": [vp, C.c_uint32, C.c_uint32, fp, C.c_float, C.c_uint32,
                                       C.POINTER(vp)],
             "dsim_waveforms_destroy": [vp],
+            "dsim_waveforms_set_buffer": [vp, C.c_uint32],
             "dsim_waveforms_run": [vp, C.c_uint32, fp],
             "dsim_waveforms_signal": [vp, dp],
             "dsim_waveforms_get_phase": [vp, fp, fp],
@@ -45,7 +46,7 @@ class Waveforms:
     """Phases of n_wave arbitrary gradient waveforms over one walk of `sim`.
     grads: [n_steps, n_wave, 3] gradScaled values."""
 
-    def __init__(self, sim, grads, phase_constant, chunk_steps=256):
+    def __init__(self, sim, grads, phase_constant, chunk_steps=256, buffer_steps=None):
         g = np.ascontiguousarray(grads, np.float32)
         self.n_steps, self.n_wave = g.shape[0], g.shape[1]
         self.n = sim.n
@@ -54,6 +55,8 @@ class Waveforms:
         _check(_lib().dsim_waveforms_create(sim.h, self.n_wave, self.n_steps,
                                             g.ctypes.data_as(C.POINTER(C.c_float)),
                                             float(phase_constant), chunk_steps, C.byref(self.h)))
+        if buffer_steps is not None:
+            _check(_lib().dsim_waveforms_set_buffer(self.h, int(buffer_steps)))
 
     def close(self):
         if self.h:

@@ -30,20 +30,26 @@ extern "C" {
 
 typedef struct dsim_waveforms dsim_waveforms;
 
-/* n_wave: a multiple of 128; n_steps and chunk_steps: multiples of 32 (a
- * chunk of 256 steps gives the GEMM K = 768; the last chunk may be shorter;
- * longer chunks read and write the phase pairs fewer times: 256 steps take
- * the 4e6-spin x 1024-waveform product 1.8 times faster than 64 steps);
- * grads: host array
+/* n_wave: a multiple of 128; n_steps and chunk_steps: multiples of 32.
+ * chunk_steps is the Kahan interval: the products of each chunk (GEMM K =
+ * 3 * chunk_steps) are summed in FP32 and then added to the phase with
+ * compensation; the last chunk of a run may be shorter.  grads: host array
  * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
  * phase_constant = 2*pi*gamma*dt.  Device memory: 8 bytes per (spin,
  * waveform) for the phase and its compensation, plus 4 * n_wave * 3 * n_steps
- * bytes of bf16 gradients and one chunk of trajectory. */
+ * bytes of bf16 gradients and one trajectory buffer (12 bytes per spin and
+ * buffered step). */
 int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_steps, const float *grads,
                           float phase_constant, uint32_t chunk_steps, dsim_waveforms **out);
 int dsim_waveforms_destroy(dsim_waveforms *wf);
+/* Steps walked into the trajectory buffer before each GEMM launch (a
+ * multiple of chunk_steps; chunk_steps at creation).  One launch folds every
+ * chunk of the buffer into phases held in registers, so the phase pairs are
+ * read and written once per buffer rather than once per chunk; the results
+ * do not depend on it. */
+int dsim_waveforms_set_buffer(dsim_waveforms *wf, uint32_t buffer_steps);
 /* Walks `steps` more steps (a multiple of 32; the total may not pass n_steps)
- * and accumulates the phases, chunk_steps at a time.  traj_out (may be NULL): host array
+ * and accumulates the phases, one trajectory buffer at a time.  traj_out (may be NULL): host array
  * [steps][3][n_spins] that receives the recorded displacements (for checks). */
 int dsim_waveforms_run(dsim_waveforms *wf, uint32_t steps, float *traj_out);
 /* Signals per waveform, [n_wave*5]: total, then compartments 0-3, as

@@ -108,3 +108,26 @@ def test_pgse_waveform_matches_moment_path(env):
     ref_sim.close()
     assert want[:, 0].min() < 0.99     # the gradients attenuate
     np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+
+
+def test_trajectory_buffer_changes_nothing(env):
+    # the Kahan folds fall every chunk whatever the buffer: one launch over a
+    # buffer of several chunks gives the phases of one launch per chunk, bit
+    # for bit (here 320 steps: a buffer of 256 steps, then one of 64)
+    n, n_wave, steps, chunk = 3000, 256, 320, 64
+    pc = np.float32(2 * np.pi * GAMMA * env["cfg"].dt)
+    g = PH.random_waveforms(n_wave, steps, seed=12, g_max=1600.0) * np.float32(
+        1e-9 * env["space_scale"])
+    out = []
+    for buffer in (64, 256):
+        sim = _sim(env, n, seed=22)
+        wf = PH.Waveforms(sim, g, pc, chunk_steps=chunk, buffer_steps=buffer)
+        wf.run(steps)
+        out.append(wf.phase() + (wf.signal(),))
+        wf.close()
+        sim.close()
+    (p0, c0, s0), (p1, c1, s1) = out
+    np.testing.assert_array_equal(p0, p1)
+    np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_array_equal(s0, s1)
+    assert np.abs(p0).max() > 0.1
