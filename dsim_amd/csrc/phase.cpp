@@ -29,6 +29,10 @@ struct dsim_waveforms {
     DevBuf<float2> phi;           // [n][n_wave]: (phase, Kahan compensation)
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
+    ~dsim_waveforms() {
+        g_hi.release(); g_lo.release(); gsum.release(); traj.release(); phi.release();
+        partial.release(); out.release();
+    }
 };
 
 namespace {
