@@ -472,6 +472,58 @@ def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=1024):
                            "signal_err_median": float(np.median(dsig))}}
 
 
+def footprint_bench(axon_counts, n=1_000_000, cell_um=0.2, sort_interval=1024):
+    """The at-scale walk on the same packing statistics in voxels of each axon
+    count (the voxel sized for the same intra-axonal fraction): smaller voxels
+    hold the whole substrate in cache, so their rate, with the search work
+    per step from the walk counters, bounds what moving the triangle records
+    nearer the compute units can win.  One acquisition each after a warm-up."""
+    import dataclasses
+
+    out = []
+    for na in axon_counts:
+        cfg = dataclasses.replace(S.SCALE, n_axons=int(na), name=f"scale-{na}axon")
+        env = S.build(cfg, cell_um=cell_um, axons=S.pack(cfg))
+        sub, sim = AT.make_sim(env, n, sort_interval=sort_interval,
+                               steps_per_launch=AT.launch_steps(sort_interval))
+        sim.init_uniform(1.0)
+        sim.label(env["xmax"], env["xmax"], env["xmax"])
+        sched = [(1.0, 2000), (0.0, 6000), (-1.0, 2000), (0.0, 2000)]
+
+        def acq():
+            sim.moment_reset()
+            for w, k in sched:
+                sim.integrate_moment(w, k)
+
+        acq()
+        sim.sync()
+        sim.step_timing_reset()
+        t0 = time.perf_counter()
+        acq()
+        sim.sync()
+        wall = time.perf_counter() - t0
+        ms, launches = sim.step_timing()
+        sim.walk_stats(True)
+        acq()
+        sim.sync()
+        st = sim.walk_stats()
+        sim.walk_stats(False)
+        k = st["steps"]
+        r = {"axons": int(na), "voxel_um": round(env["axons"].voxel, 2),
+             "triangles": len(env["triangles"]), "substrate_bytes": sub.nbytes(),
+             "spin_steps_per_s": n * 12000 / wall, "kernel_ms_per_launch": ms / max(1, launches),
+             "per_spin_step": {key: st[key] / k for key in ("trips", "plane_tests", "tri_tests",
+                                                              "hits", "shortcuts", "cells")}}
+        r["per_spin_step"]["trips"] *= 64  # wave trips per lane-step
+        log(f"footprint: {na} axons, voxel {r['voxel_um']} um, {r['substrate_bytes'] / 2**20:.1f} "
+            f"MiB of substrate: {r['spin_steps_per_s']:.4e} spin-steps/s, plane tests "
+            f"{r['per_spin_step']['plane_tests']:.3f} per spin-step")
+        out.append(r)
+        sim.close()
+        sub.close()
+    return out
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walk_scale-2000axon-1e6_hbm_traffic.md")
 
 
@@ -557,6 +609,8 @@ def main():
                     help="Kahan interval of the waveform phases in steps (multiple of 32)")
     ap.add_argument("--buffer-steps", type=int, default=1024,
                     help="steps walked per phase-GEMM launch (multiple of --chunk-steps)")
+    ap.add_argument("--footprint-axons", default="", metavar="N,N,...",
+                    help="only the substrate-footprint measurement at these axon counts")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -567,6 +621,10 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.footprint_axons:
+        print(json.dumps({"footprint": footprint_bench(
+            [int(a) for a in args.footprint_axons.split(",")])}))
         return
     if D.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)

@@ -1,5 +1,5 @@
 // dsim_example: the reference host's call sequence on libdsim, through the
-// SpinSystem mirror (spin_system.h) that INTEGRATION.md describes:
+// SpinSystem mirror (spin_system.h):
 // initFibers, populateCubes, build, setArray, findSpinsInFibers, getArray,
 // setGradient, updateSpins, getArray, getMrSignal
 // (dSim3/src/spinSystem.cpp:290-673, 1094-1352, 178-281, 1584-1606,

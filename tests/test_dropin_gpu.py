@@ -3,8 +3,9 @@ reference's `diffusion` program, diffusion.cpp:1068-1184) on a sample mesh
 and the reference's sample sim.grads, its .m output against the layout of
 the reference's own sample output (matlab_tests/data_outputs/testfile.m,
 written from the same sim.grads) and its signals against the oracle; and the
-SpinSystem call sequence of INTEGRATION.md (dsim_example) against the
-oracle."""
+SpinSystem call sequence of the reference's host (dsim_example: initFibers,
+populateCubes, build, setArray, findSpinsInFibers, getArray, updateSpins,
+getMrSignal) against the oracle."""
 import os
 import re
 import subprocess
@@ -177,7 +178,7 @@ def test_cli_output_layout_and_signals(tmp_path, shared):
 
 
 def test_spin_system_sequence_matches_oracle(tmp_path):
-    # the INTEGRATION.md call sequence through dsim::SpinSystem
+    # the reference host's call sequence through dsim::SpinSystem
     n, iters = 4000, 200
     out = str(tmp_path / "state.bin")
     r = subprocess.run([os.path.join(BIN, "dsim_example"), MESH, str(SPACE), str(n), str(iters),
