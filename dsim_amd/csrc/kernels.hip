@@ -952,7 +952,17 @@ __device__ __forceinline__ uint32_t pack_wrap(int kx, int ky, int kz) {
 // bounds), or a periodic box with the bounds of the launch parameters.
 enum { kBoundClamp = 0, kBoundCube = 1, kBoundBox = 2 };
 
-template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS>
+// Spin slots of the persistent walk: queue q (one per XCD) hands out the
+// slots of runs q, q + 8, q + 16, ... of kWalkQueueRun consecutive slots, so
+// that every XCD walks a like mix of the sorted order and a compact stretch
+// of it at a time.  Returns n (no slot) past the end.
+constexpr uint32_t kWalkQueueRun = 8192;
+__device__ __forceinline__ uint32_t queue_slot(uint32_t q, uint32_t g, uint32_t n) {
+    const uint64_t s = ((uint64_t)(g / kWalkQueueRun) * kXcds + q) * kWalkQueueRun + g % kWalkQueueRun;
+    return s < n ? (uint32_t)s : n;
+}
+
+template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS, bool QUEUE = false>
 __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
                                           WaveScratch &L) {
     constexpr bool PERIODIC = BOUND != kBoundClamp;
@@ -962,16 +972,14 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         for (uint32_t k = 0; k < kWalkStats; ++k) C.v[k] = 0;
     uint32_t shortcuts = 0;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t i = block * blockDim.x + threadIdx.x;
-    // lanes past the last spin stay in the loop (done) for the wave operations
-    const bool live = i < P.n;
     uint32_t sid = 0, comp = 0, fiber = kFiberNone, s0 = 0, s1 = 0;
     float x = 0.f, y = 0.f, z = 0.f, mag = 0.f, ph = 0.f;
     float dx = 0.f, dy = 0.f, dz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     float safe = -1.0f;
     // voxel crossings (periodic phase): the unwrapped position is x + kx * period
     int kx = 0, ky = 0, kz = 0;
-    if (live) {
+    uint32_t nhit = 0, hit_total = 0;
+    auto load = [&](uint32_t i) {
         sid = P.spin_id ? P.spin_id[i] : i;
         x = P.x[i]; y = P.y[i]; z = P.z[i];
         mag = P.mag[i];
@@ -986,13 +994,44 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             cx = P.momc[i]; cy = P.momc[P.n + i]; cz = P.momc[2 * P.n + i];
         }
         if (P.cell_clear) safe = clearance_at(P, x, y, z);
-    }
+        nhit = 0;
+    };
+    auto store = [&](uint32_t i) {
+        P.x[i] = x; P.y[i] = y; P.z[i] = z;
+        P.mag[i] = mag;
+        if (!MOMENT) P.phase[i] = ph;
+        if (PERIODIC && !MOMENT) P.wrap[i] = pack_wrap(kx, ky, kz);
+        P.label[i] = comp | (fiber << 8);
+        if (RNG == kRngMwc) { P.seed[2 * i] = s0; P.seed[2 * i + 1] = s1; }
+        if (MOMENT) {
+            P.disp[i] = dx; P.disp[P.n + i] = dy; P.disp[2 * P.n + i] = dz;
+            P.mom[i] = mx; P.mom[P.n + i] = my; P.mom[2 * P.n + i] = mz;
+            P.momc[i] = cx; P.momc[P.n + i] = cy; P.momc[2 * P.n + i] = cz;
+        }
+        if (P.hits) P.hits[i] += nhit;
+        if (STATS) hit_total += nhit;
+    };
+    // The persistent walk (QUEUE): lanes take slots from their XCD's queue,
+    // the finishing lanes of a trip together (one atomic per wave).
+    const uint32_t q = block % kXcds;
+    auto take = [&](bool want) -> uint32_t {
+        const unsigned long long m = __ballot(want);
+        uint32_t base = 0;
+        const uint32_t lead = (uint32_t)__builtin_ctzll(m);
+        if (lane == lead) base = atomicAdd(P.queue + q, popc(m));
+        base = __builtin_amdgcn_readlane(base, lead);
+        return want ? queue_slot(q, base + lane_rank(m), P.n) : P.n;
+    };
+    // lanes past the last spin stay in the loop (done) for the wave operations
+    uint32_t i = QUEUE ? take(true) : block * blockDim.x + threadIdx.x;
+    bool live = i < P.n;
+    if (live) load(i);
     uint32_t it = live ? 0u : iters;
     // the pending segment of the current step and its permeation interval
     bool pending = false, refresh = false;
     float ox = 0.f, oy = 0.f, oz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     float u = 0.f, umax = 1.f, umin = 0.f;
-    uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0, nhit = 0;
+    uint32_t excluded = kNone, first = kNone, last = kNone, ncoll = 0;
     uint32_t slice = 0;  // cells of the segment's box searched so far
     while (__ballot(it < iters) != 0ull) {
         bool complete = false;
@@ -1136,24 +1175,29 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             }
             ++it;
         }
-    }
-    if (live) {
-        P.x[i] = x; P.y[i] = y; P.z[i] = z;
-        P.mag[i] = mag;
-        if (!MOMENT) P.phase[i] = ph;
-        if (PERIODIC && !MOMENT) P.wrap[i] = pack_wrap(kx, ky, kz);
-        P.label[i] = comp | (fiber << 8);
-        if (RNG == kRngMwc) { P.seed[2 * i] = s0; P.seed[2 * i + 1] = s1; }
-        if (MOMENT) {
-            P.disp[i] = dx; P.disp[P.n + i] = dy; P.disp[2 * P.n + i] = dz;
-            P.mom[i] = mx; P.mom[P.n + i] = my; P.mom[2 * P.n + i] = mz;
-            P.momc[i] = cx; P.momc[P.n + i] = cy; P.momc[2 * P.n + i] = cz;
+        if (QUEUE) {
+            // a lane whose spin has walked its steps stores it and takes the
+            // next slot of its queue
+            const bool fin = live && it == iters;
+            if (__ballot(fin) != 0ull) {
+                if (fin) store(i);
+                if (STATS) C.v[kStatSteps] += popc(__ballot(fin)) * iters;
+                const uint32_t ni = take(fin);
+                if (fin) {
+                    i = ni;
+                    live = i < P.n;
+                    if (live) {
+                        load(i);
+                        it = 0;
+                    }
+                }
+            }
         }
-        if (P.hits) P.hits[i] += nhit;
     }
+    if (!QUEUE && live) store(i);
     if (STATS) {
-        C.v[kStatSteps] = popc(__ballot(live)) * iters;
-        C.v[kStatHits] = __builtin_amdgcn_readlane(wave_incl_sum(nhit), 63);
+        if (!QUEUE) C.v[kStatSteps] = popc(__ballot(live)) * iters;
+        C.v[kStatHits] = __builtin_amdgcn_readlane(wave_incl_sum(hit_total), 63);
         C.v[kStatShortcut] = __builtin_amdgcn_readlane(wave_incl_sum(shortcuts), 63);
         C.v[kStatWaves] = 1;
         if (lane == 0)
@@ -1170,6 +1214,18 @@ __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu
     StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[1];
     walk_body<BOUND, RNG, INSTR, MOMENT, RECS>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
+}
+
+// Persistent walk: one one-wave workgroup per resident slot (block b on XCD
+// b mod 8); each lane walks a spin's steps, stores it and takes the next slot
+// from its XCD's queue, so no lane idles while its wave waits for the slowest
+// spin of a fixed block.  Spins are independent (Philox counters keyed by the
+// spin id), so the results equal k_walk's.
+template <int BOUND, int INSTR, bool MOMENT, int RECS>
+__global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_queue(
+    StepParams P, uint32_t iters) {
+    __shared__ WaveScratch scratch[1];
+    walk_body<BOUND, kRngPhilox, INSTR, MOMENT, RECS, true>(P, blockIdx.x, iters, scratch[0]);
 }
 
 // Batched walk for the server: the blocks of one launch are shared out over
@@ -1209,8 +1265,36 @@ hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool 
     return hipGetLastError();
 }
 
+// Resident one-wave workgroups of the walk on this device (four per SIMD).
+static uint32_t resident_walk_blocks() {
+    static uint32_t cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        cached = (uint32_t)cus * 16u;
+    }
+    return cached;
+}
+
 template <int MODE, int BOUND, int RNG, int INSTR, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
+    if constexpr (MODE == 1 && RNG == kRngPhilox && INSTR != kInstrTrace) {
+        // the persistent walk once every resident slot has two spins or more
+        const uint32_t slots = resident_walk_blocks();
+        if (p.queue && p.n >= 2 * slots * kWalkBlockSpins) {
+            if (hipMemsetAsync(p.queue, 0, kXcds * sizeof(uint32_t), s) != hipSuccess) return;
+            if (p.cell_mt)
+                hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsCellMajor>), dim3(slots),
+                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+            else
+                hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsIndirect>), dim3(slots),
+                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+            return;
+        }
+    }
     if constexpr (MODE == 0) {
         hipLaunchKernelGGL((k_step<0, BOUND != kBoundClamp, RNG, INSTR == kInstrTrace, MOMENT>),
                            dim3((p.n + 63) / 64), dim3(64), 0, s, p, iters);

@@ -52,6 +52,7 @@ struct dsim_sim {
     DevBuf<float4> inject;
     DevBuf<uint32_t> trace;
     DevBuf<unsigned long long> stats;  // walk work counters while counting is on
+    DevBuf<uint32_t> queue;            // slot counters of the persistent walk
     bool counting = false;
     DevBuf<uint32_t> keys_a, keys_b, vals_a, vals_b;
     DevBuf<uint8_t> sort_tmp;
@@ -74,7 +75,7 @@ struct dsim_sim {
                         &vals_b})
             b->release();
         partial.release(); out.release(); grads.release(); inject.release(); sort_tmp.release();
-        stats.release();
+        stats.release(); queue.release();
         if (stream) (void)hipStreamDestroy(stream);
         if (sub) sub->refs--;
     }
@@ -125,7 +126,7 @@ extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32
     }
     const size_t m = std::max<uint32_t>(n, 1);
     int rc;
-    if ((rc = S->state.alloc(m * F_COUNT)) || (rc = S->flags.alloc(4)) ||
+    if ((rc = S->state.alloc(m * F_COUNT)) || (rc = S->flags.alloc(4)) || (rc = S->queue.alloc(8)) ||
         (rc = S->partial.alloc((size_t)dsim::kSignalBlocks * 15)) || (rc = S->out.alloc(15)) ||
         (rc = S->tries.alloc(m)) ||
         (p->rng == DSIM_RNG_MWC && (rc = S->seed.alloc(2 * m)))) {
@@ -351,6 +352,7 @@ static dsim::StepParams step_params(dsim_sim *s) {
     P.key0 = (uint32_t)s->prm.seed;
     P.key1 = (uint32_t)(s->prm.seed >> 32);
     P.flags = s->flags.p;
+    P.queue = s->prm.fixed_blocks ? nullptr : s->queue.p;
     return P;
 }
 

@@ -64,7 +64,7 @@ class Params(C.Structure):
                 ("boundary", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint64),
                 ("spin_radius", C.c_float), ("sort_interval", C.c_uint32),
                 ("steps_per_launch", C.c_uint32), ("voxel_lo", C.c_float * 3),
-                ("voxel_hi", C.c_float * 3)]
+                ("voxel_hi", C.c_float * 3), ("fixed_blocks", C.c_uint32)]
 
 
 _lib = None
@@ -265,7 +265,8 @@ class Simulation:
 
     def __init__(self, substrate: Substrate, n, *, dt, std_dev, t2=(np.inf,) * 4, permeability=0.0,
                  reflection_type=1, mode=MODE_REFERENCE, boundary=BOUNDARY_CLAMP, rng=RNG_PHILOX,
-                 seed=1, spin_radius=0.005, sort_interval=0, steps_per_launch=0, voxel=None):
+                 seed=1, spin_radius=0.005, sort_interval=0, steps_per_launch=0, voxel=None,
+                 fixed_blocks=False):
         self.substrate = substrate
         self.n = int(n)
         p = Params()
@@ -281,6 +282,7 @@ class Simulation:
         p.spin_radius = float(spin_radius)
         p.sort_interval = int(sort_interval)
         p.steps_per_launch = int(steps_per_launch)
+        p.fixed_blocks = int(bool(fixed_blocks))
         if voxel is not None:  # ((lo x, y, z), (hi x, y, z)) of a non-cubic periodic voxel
             p.voxel_lo = (C.c_float * 3)(*[float(v) for v in voxel[0]])
             p.voxel_hi = (C.c_float * 3)(*[float(v) for v in voxel[1]])

@@ -110,6 +110,10 @@ typedef struct {
                                   most sort_interval) */
     float voxel_lo[3];        /* periodic voxel [voxel_lo, voxel_hi) per axis, normalised;     */
     float voxel_hi[3];        /* all six zero means [-1, 1) on every axis (the cubic voxel)    */
+    uint32_t fixed_blocks;    /* physical mode with Philox: 0 lets a persistent walk take over
+                               * once every resident workgroup has two or more spins (each lane
+                               * takes its next spin as soon as one is done); 1 keeps one
+                               * 64-spin workgroup per block.  Results do not depend on it. */
 } dsim_params;
 
 typedef struct dsim_substrate dsim_substrate;

@@ -112,3 +112,39 @@ def test_counting_refused_for_mwc(ref_inputs):
     sim.walk_stats(True)
     with pytest.raises(D.DsimError):
         sim.integrate(np.zeros(3, np.float32), 0.0, 1)
+
+
+def test_persistent_walk_equals_fixed_blocks(ref_inputs):
+    # 6e5 spins: at least two per resident one-wave workgroup, so the
+    # persistent walk (lanes taking their next spin from a per-XCD queue)
+    # runs; its results and counts equal the fixed-block walk's
+    _say("case persistent walk")
+    n, iters = 600_000, 200
+    m = O.parse_mesh(os.path.join(ref_inputs, "sim.fibers_1_r004_f09"), 6.0)
+    dims, lo, cell = (37, 29, 13), (-1.2, -1.2, -1.2), (2.4 / 37, 2.4 / 29, 2.4 / 13)
+    sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims, lo=lo,
+                      cell=cell, eps=1e-5)
+    out = []
+    for fixed in (True, False):
+        sim = D.Simulation(sub, n, dt=DT, std_dev=(0.03, 0.03, 0.01, 0.03),
+                           t2=(80.0, 70.0, 7.5, 70.0), permeability=0.2, mode=D.MODE_PHYSICAL,
+                           boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=7,
+                           sort_interval=64, fixed_blocks=fixed)
+        sim.init_uniform(0.9)
+        sim.label(m["xmax"], m["ymax"], m["zmax"])
+        sim.moment_reset()
+        sim.walk_stats(True)
+        sim.integrate_moment(1.0, iters)
+        g = sim.get_state()
+        mom, _ = sim.get_moment()
+        out.append((g, mom, sim.walk_stats()))
+        sim.close()
+    (g0, m0, c0), (g1, m1, c1) = out
+    for k in ("pos", "mag", "comp", "fiber"):
+        np.testing.assert_array_equal(g0[k], g1[k])
+    np.testing.assert_array_equal(m0, m1)
+    _say(f"  fixed {c0}\n  persistent {c1}")
+    assert c0["steps"] == c1["steps"] == n * iters
+    assert c0["hits"] == c1["hits"] > 0
+    assert c1["waves"] < c0["waves"]
+    _consistent(c1, n * iters)
