@@ -191,9 +191,12 @@ struct Fields {
     uint32_t count;
 };
 hipError_t sort_temp_bytes(uint32_t n, size_t *bytes);
-hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *keys_out,
-                            uint32_t *vals_in, uint32_t *vals_out, void *temp, size_t temp_bytes,
-                            hipStream_t s);
+// Sort keys of the spins (hit band, then cell Morton index) and the slot
+// permutation that orders them, stably; *perm points at the permutation (one
+// of the two value buffers).
+hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_a, uint32_t *keys_b,
+                            uint32_t *vals_a, uint32_t *vals_b, void *temp, size_t temp_bytes,
+                            hipStream_t s, const uint32_t **perm);
 hipError_t launch_gather(uint32_t n, const uint32_t *perm, const Fields &f, hipStream_t s);
 
 }  // namespace dsim

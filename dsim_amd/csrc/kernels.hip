@@ -6,7 +6,6 @@
 // path's hit triangles, cells, labels, positions and phases bit for bit; the
 // physical-mode code spells its fused multiply-adds out with fmaf().
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstdint>
 
@@ -1282,9 +1281,11 @@ static uint32_t resident_walk_blocks() {
 template <int MODE, int BOUND, int RNG, int INSTR, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     if constexpr (MODE == 1 && RNG == kRngPhilox && INSTR != kInstrTrace) {
-        // the persistent walk once every resident slot has two spins or more
+        // the persistent walk once every resident slot has two spins or more;
+        // not while a trajectory is recorded (its lanes, at different steps of
+        // different spins, would scatter every trajectory store over 64 rows)
         const uint32_t slots = resident_walk_blocks();
-        if (p.queue && p.n >= 2 * slots * kWalkBlockSpins) {
+        if (p.queue && !p.traj && p.n >= 2 * slots * kWalkBlockSpins) {
             if (hipMemsetAsync(p.queue, 0, kXcds * sizeof(uint32_t), s) != hipSuccess) return;
             if (p.cell_mt)
                 hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsCellMajor>), dim3(slots),
@@ -2237,20 +2238,130 @@ __global__ __launch_bounds__(256) void k_sort_key(StepParams P, uint32_t *keys, 
     vals[i] = i;
 }
 
-hipError_t sort_temp_bytes(uint32_t n, size_t *bytes) {
-    return rocprim::radix_sort_pairs(nullptr, *bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
+// Spin reorder: a stable LSD radix sort of the 32-bit keys with the slot
+// index as value, four passes of 8-bit digits.  Each pass: per-tile digit
+// histograms (tiles of 4096 keys), one exclusive scan over them in
+// digit-major order (every tile's count of digit 0, then of digit 1, ...),
+// then each tile scatters its keys to its digit offsets, ranking equal digits
+// in key order.  Within a wave the lanes holding the same digit are found
+// with one ballot per digit bit; the waves of a tile combine their counts
+// through LDS.
+constexpr uint32_t kSortThreads = 256, kSortPerThread = 16;
+constexpr uint32_t kSortTile = kSortThreads * kSortPerThread;  // keys per tile
+constexpr uint32_t kRadix = 256;
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys,
+                                                             uint32_t n, uint32_t shift,
+                                                             uint32_t n_tiles,
+                                                             uint32_t *__restrict__ hist) {
+    __shared__ uint32_t cnt[kRadix];
+    const uint32_t t = threadIdx.x, tile = blockIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kSortPerThread; ++k) {
+        const uint32_t i = tile * kSortTile + k * kSortThreads + t;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    hist[t * n_tiles + tile] = cnt[t];
 }
 
-hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_in, uint32_t *keys_out,
-                            uint32_t *vals_in, uint32_t *vals_out, void *temp, size_t temp_bytes,
-                            hipStream_t s) {
+// Exclusive scan of m values in place, one workgroup: each thread sums a
+// contiguous stretch, the stretch sums are scanned through LDS, then each
+// thread writes its stretch's prefixes.
+__global__ __launch_bounds__(kScanThreads) void k_radix_scan(uint32_t *__restrict__ v, uint32_t m) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (m + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = min(m, t * per), b1 = min(m, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += v[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < kScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t add = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t x = v[i];
+        v[i] = run;
+        run += x;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, uint32_t n, uint32_t shift,
+    uint32_t n_tiles, const uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t run[kRadix];                        // next free place per digit
+    __shared__ uint32_t wave_cnt[kSortThreads / 64][kRadix];  // per-wave digit counts
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, tile = blockIdx.x;
+    run[t] = offsets[t * n_tiles + tile];
+    for (uint32_t q = 0; q < kSortThreads / 64; ++q) wave_cnt[q][t] = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kSortPerThread; ++k) {
+        const uint32_t i = tile * kSortTile + k * kSortThreads + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t d = (key >> shift) & (kRadix - 1);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t bit = 0; bit < 8; ++bit) {
+            const unsigned long long m = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? m : ~m;
+        }
+        const uint32_t below = lane_rank(peers);
+        if (valid && below == 0) wave_cnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + below;
+            for (uint32_t q = 0; q < w; ++q) pos += wave_cnt[q][d];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t q = 0; q < kSortThreads / 64; ++q) {
+            add += wave_cnt[q][t];
+            wave_cnt[q][t] = 0;
+        }
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+hipError_t sort_temp_bytes(uint32_t n, size_t *bytes) {
+    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    *bytes = std::max<size_t>(1, tiles) * kRadix * sizeof(uint32_t);
+    return hipSuccess;
+}
+
+hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_a, uint32_t *keys_b,
+                            uint32_t *vals_a, uint32_t *vals_b, void *temp, size_t temp_bytes,
+                            hipStream_t s, const uint32_t **perm) {
+    *perm = vals_a;
     if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_key, dim3((p.n + 255) / 256), dim3(256), 0, s, p, keys_in, vals_in);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, p.n,
-                                     0, 32, s);
+    const uint32_t tiles = (p.n + kSortTile - 1) / kSortTile;
+    if (temp_bytes < (size_t)tiles * kRadix * sizeof(uint32_t)) return hipErrorInvalidValue;
+    uint32_t *hist = static_cast<uint32_t *>(temp);
+    hipLaunchKernelGGL(k_sort_key, dim3((p.n + 255) / 256), dim3(256), 0, s, p, keys_a, vals_a);
+    uint32_t *ki = keys_a, *ko = keys_b, *vi = vals_a, *vo = vals_b;
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, ki, p.n, shift,
+                           tiles, hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kScanThreads), 0, s, hist, tiles * kRadix);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
+                           p.n, shift, tiles, hist);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *perm = vi;  // four passes: back in the first buffers
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_gather(uint32_t n, const uint32_t *perm, Fields f) {

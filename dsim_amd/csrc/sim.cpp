@@ -362,8 +362,9 @@ static int spatial_sort(dsim_sim *s) {
     dsim::StepParams P = step_params(s);
     // hit counts cover the steps since the last reorder (none before the first)
     P.hit_window = s->steps_since_sort < (1ull << 31) ? (uint32_t)s->steps_since_sort : 0u;
+    const uint32_t *perm = nullptr;
     hipError_t e = dsim::launch_sort_perm(P, s->keys_a.p, s->keys_b.p, s->vals_a.p, s->vals_b.p,
-                                          s->sort_tmp.p, s->sort_tmp.n, s->stream);
+                                          s->sort_tmp.p, s->sort_tmp.n, s->stream, &perm);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "sort: %s", hipGetErrorString(e));
     dsim::Fields fl{};
     fl.count = F_COUNT;
@@ -371,7 +372,7 @@ static int spatial_sort(dsim_sim *s) {
         fl.src[k] = s->state.p + (size_t)k * s->n;
         fl.dst[k] = s->alt.p + (size_t)k * s->n;
     }
-    e = dsim::launch_gather(s->n, s->vals_b.p, fl, s->stream);
+    e = dsim::launch_gather(s->n, perm, fl, s->stream);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "gather: %s", hipGetErrorString(e));
     std::swap(s->state, s->alt);
     s->permuted = true;
