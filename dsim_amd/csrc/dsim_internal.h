@@ -149,6 +149,9 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s);
 enum { kRecsEither = 0, kRecsCellMajor = 1, kRecsIndirect = 2 };
 hipError_t launch_walk_batch(const WalkTable *table, uint32_t grid_blocks, bool periodic,
                              int recs, hipStream_t s);
+// MWC seed pairs of n spins from Philox stream 3 (both words nonzero).
+hipError_t launch_init_seeds(uint32_t n, uint32_t *seed, uint32_t key0, uint32_t key1,
+                             hipStream_t s);
 hipError_t launch_init_uniform(uint32_t n, float *x, float *y, float *z, float *mag, float *phase,
                                uint32_t *label, uint32_t *id, uint32_t *wrap, const float lo[3],
                                const float hi[3], uint32_t key0, uint32_t key1, uint64_t draw,

@@ -2146,6 +2146,24 @@ hipError_t launch_label_row(const LabelParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// MWC seed pairs for spin i: the first two words of Philox block (i, 0, 0, 3);
+// a zero word (which would stall its multiply-with-carry half) becomes 1.
+__global__ __launch_bounds__(256) void k_init_seeds(uint32_t n, uint32_t *seed, uint32_t k0,
+                                                    uint32_t k1) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint4 r = philox(make_uint4(i, 0u, 0u, 3u), k0, k1);
+    seed[2 * i] = r.x ? r.x : 1u;
+    seed[2 * i + 1] = r.y ? r.y : 1u;
+}
+
+hipError_t launch_init_seeds(uint32_t n, uint32_t *seed, uint32_t key0, uint32_t key1,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_seeds, dim3((n + 255) / 256), dim3(256), 0, s, n, seed, key0, key1);
+    return hipGetLastError();
+}
+
 // resetSpins position draw (spinSystem.cpp:1386-1398) from Philox stream 2.
 // Per axis: width * (u - 0.5) + centre, u in (0, 1] (the box [-b, b] has
 // width 2b and centre 0, which leaves resetSpins' expression unchanged).

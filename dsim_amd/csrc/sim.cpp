@@ -149,7 +149,12 @@ extern "C" int dsim_sim_create(dsim_substrate *sub, const dsim_params *p, uint32
     }
     e = hipMemsetAsync(S->state.p, 0, m * F_COUNT * 4, S->stream);
     if (e == hipSuccess) e = hipMemsetAsync(S->flags.p, 0, 16, S->stream);
-    if (e == hipSuccess && S->seed.p) e = hipMemsetAsync(S->seed.p, 0, 2 * m * 4, S->stream);
+    // MWC seeds: a draw keyed by the Philox key, so a fresh handle walks a
+    // random stream (an all-zero MWC state repeats one draw forever); callers
+    // with the reference's seeds replace them with dsim_set_state_aos
+    if (e == hipSuccess && S->seed.p)
+        e = dsim::launch_init_seeds(n, S->seed.p, (uint32_t)p->seed, (uint32_t)(p->seed >> 32),
+                                    S->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
     if (e != hipSuccess) {
         delete S;

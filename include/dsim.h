@@ -100,7 +100,9 @@ typedef struct {
     uint32_t reflection_type; /* 1 = specular (diffusion.cpp:96), 0 = back to start */
     uint32_t mode;            /* DSIM_MODE_* */
     uint32_t boundary;        /* DSIM_BOUNDARY_*; the reference mode always clamps */
-    uint32_t rng;             /* DSIM_RNG_*; MWC is the reference's per-spin stream */
+    uint32_t rng;             /* DSIM_RNG_*; MWC is the reference's per-spin stream (its seed
+                               * pairs start as a draw keyed by `seed`, both words nonzero;
+                               * dsim_set_state_aos sets the reference's own) */
     uint64_t seed;            /* Philox key */
     float spin_radius;        /* clamp margin (spinSystem.cpp:61: 0.005) */
     uint32_t sort_interval;   /* physical mode with Philox: reorder spins in device memory

@@ -435,3 +435,71 @@ def test_sorted_philox_run_is_order_independent(ref_inputs):
         np.testing.assert_array_equal(ma, mb)
         np.testing.assert_array_equal(xa, xb)
         np.testing.assert_allclose(sa, sb, rtol=1e-12)
+
+
+def test_fresh_mwc_handle_walks_a_random_stream():
+    # a handle created with MWC draws and no seeds of its own starts from
+    # distinct nonzero seed pairs (an all-zero multiply-with-carry state would
+    # repeat one draw every step): the free walk has zero mean and variance
+    # iters * sd^2 per axis
+    _say("case fresh mwc seeds")
+    V = np.zeros((3, 3), np.float32) + 5
+    n, iters, sd = 20000, 100, 0.01
+    w, sub, sim = _pair({"vertices": V, "trgls": [[0, 1, 2]], "tri_info": [[0, 0, 0]]}, n=n,
+                        rng=D.RNG_MWC, std=(sd,) * 4, seed=9)
+    sim.init_uniform(0.5)
+    g0 = sim.get_state(seeds=True)
+    seeds = g0["seeds"]
+    assert (seeds != 0).all()
+    assert len(np.unique(seeds.view(np.uint64))) == n
+    sim.integrate(np.zeros(3, np.float32), 0.0, iters)
+    d = (sim.get_state()["pos"] - g0["pos"]).astype(np.float64)
+    se = sd * np.sqrt(iters / n)
+    assert np.all(np.abs(d.mean(0)) < 5 * se), d.mean(0)
+    np.testing.assert_allclose(d.var(0), iters * sd * sd, rtol=0.05)
+
+
+def test_reference_mode_stops_a_step_that_never_ends():
+    # the step of the small configuration on which the reference's collision
+    # loop never ends (tests/test_oracle_pins.py): the kernel stops the spin
+    # after 4096 collisions, the call reports DSIM_ERR_OVERFLOW once, the
+    # state stays valid and the walk continues; in the physical mode the same
+    # step ends and matches the oracle bit for bit
+    _say("case never-ending reference step")
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_pins import _shared_diagonal_case
+    env, pos, q = _shared_diagonal_case()
+    V, T, I = env["vertices"], env["triangles"], env["tri_info"]
+    sub = D.Substrate(V, T, I)
+    g0 = np.zeros(3, np.float32)
+    sim = D.Simulation(sub, 1, dt=DT, std_dev=env["std_dev"], t2=env["t2"], mode=D.MODE_REFERENCE,
+                       rng=D.RNG_PHILOX)
+    sim.set_state(pos=pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    with pytest.raises(D.DsimError) as err:
+        sim.integrate_injected(g0, 0.0, q, trace=True)
+    assert err.value.code == -5                  # DSIM_ERR_OVERFLOW
+    g = sim.get_state()                          # the state is readable again
+    assert np.all(np.abs(g["pos"]) <= 1.0)
+    q2 = np.array([[[0.1, 0.2, 0.3, 0.5]]], np.float32)
+    sim.integrate_injected(g0, 0.0, q2)          # and the walk goes on
+    sim.sync()
+    sim.close()
+    w = O.World(V, T, I, std_dev=env["std_dev"], t2=env["t2"], dt=DT, permeability=0.0)
+    w.set(test=1, traversal=1, transitions=1)
+    st = O.State(pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    tr_o = O.integrate(w, st, 1, rng=O.RNG_INJECT, inject=q, trace=True)
+    dims, lo, cell = (50, 50, 50), (-1.0, -1.0, -1.0), (0.04, 0.04, 0.04)
+    sub2 = D.Substrate(V, T, I, grid="aabb", dims=dims, lo=lo, cell=cell, eps=1e-5)
+    w2 = O.World(V, T, I, std_dev=env["std_dev"], t2=env["t2"], dt=DT, permeability=0.0,
+                 grid="aabb", dims=dims, lo=lo, cell=cell, eps=1e-5)
+    w2.set(test=1, traversal=1, transitions=1)
+    st2 = O.State(pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    tr_o2 = O.integrate(w2, st2, 1, rng=O.RNG_INJECT, inject=q, trace=True)
+    sim = D.Simulation(sub2, 1, dt=DT, std_dev=env["std_dev"], t2=env["t2"],
+                       mode=D.MODE_PHYSICAL, rng=D.RNG_PHILOX)
+    sim.set_state(pos=pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    tr_g = sim.integrate_injected(g0, 0.0, q, trace=True)
+    assert tr_o[0, 0, 2] < 10 and tr_o2[0, 0, 2] < 10
+    np.testing.assert_array_equal(tr_g[..., [0, 1, 2, 4]], tr_o2[..., [0, 1, 2, 4]])
+    np.testing.assert_array_equal(sim.get_state()["pos"], st2.pos)
+    sim.close()

@@ -670,3 +670,41 @@ def test_find_spins_plus_x_mt_on_a_packing():
     frac = np.mean(comp == 1)
     sigma = np.sqrt(ax.intra_fraction * (1 - ax.intra_fraction) / n)
     assert abs(frac - ax.intra_fraction) < 3 * sigma + 0.01, (frac, ax.intra_fraction)
+
+
+def _shared_diagonal_case():
+    """A step of the small configuration's walk (16 capped 32-gon prisms, the
+    reference 50^3 grid) whose segment meets a prism facet on the diagonal its
+    two coplanar triangles share; the spin's state and draw were taken from a
+    1e9-spin-step oracle run with the reference's MWC stream."""
+    from dsim_amd import substrate as S
+    cfg = S.SMALL
+    env = S.build(cfg, cell_um=0.2, axons=S.pack(cfg))
+    pos = np.array([[-0.01720589, -0.5001226, -0.9633706]], np.float32)
+    q = np.array([[[2.1228313, 0.6740799, -1.0266286, 0.646406]]], np.float32)
+    return env, pos, q
+
+
+def test_reference_collision_loop_never_ends_on_a_shared_facet_diagonal():
+    # collDetectRectGrid (spinKernelCpu.cpp:209-277) excludes only the last
+    # triangle hit: the softsurfer test reports the hit on one triangle of the
+    # facet, the reflected segment hits its coplanar twin at t = 0, and back,
+    # without end.  The oracle stops at its 100000-iteration guard.
+    env, pos, q = _shared_diagonal_case()
+    V, T, I = env["vertices"], env["triangles"], env["tri_info"]
+    w = O.World(V, T, I, std_dev=env["std_dev"], t2=env["t2"], dt=0.005, permeability=0.0)
+    st = O.State(pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    tr = O.integrate(w, st, 1, rng=O.RNG_INJECT, inject=q, trace=True)
+    assert tr[0, 0, 2] == 100000                  # the guard, not a finished step
+    assert {int(tr[0, 0, 0]), int(tr[0, 0, 1])} == {1200, 1168}
+    P0, P1 = V[T[1200]].astype(np.float64), V[T[1168]].astype(np.float64)
+    n0 = np.cross(P0[1] - P0[0], P0[2] - P0[0])
+    n1 = np.cross(P1[1] - P1[0], P1[2] - P1[0])
+    assert np.linalg.norm(np.cross(n0, n1)) < 1e-9 * np.linalg.norm(n0) * np.linalg.norm(n1)
+    assert len(set(T[1200].tolist()) & set(T[1168].tolist())) == 2   # one shared edge
+    # the physical mode's strict side test does not count a segment that
+    # starts on the plane: the same step ends
+    w.set(test=1, traversal=1, transitions=1)
+    st = O.State(pos, comp=np.zeros(1, np.uint8), fiber=np.full(1, 65535, np.uint16))
+    tr = O.integrate(w, st, 1, rng=O.RNG_INJECT, inject=q, trace=True)
+    assert tr[0, 0, 2] < 10
