@@ -2267,7 +2267,6 @@ __global__ __launch_bounds__(256) void k_sort_key(StepParams P, uint32_t *keys, 
 constexpr uint32_t kSortThreads = 256, kSortPerThread = 16;
 constexpr uint32_t kSortTile = kSortThreads * kSortPerThread;  // keys per tile
 constexpr uint32_t kRadix = 256;
-constexpr uint32_t kScanThreads = 1024;
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys,
                                                              uint32_t n, uint32_t shift,
@@ -2285,30 +2284,55 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__r
     hist[t * n_tiles + tile] = cnt[t];
 }
 
-// Exclusive scan of m values in place, one workgroup: each thread sums a
-// contiguous stretch, the stretch sums are scanned through LDS, then each
-// thread writes its stretch's prefixes.
-__global__ __launch_bounds__(kScanThreads) void k_radix_scan(uint32_t *__restrict__ v, uint32_t m) {
-    __shared__ uint32_t part[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (m + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = min(m, t * per), b1 = min(m, b0 + per);
-    uint32_t sum = 0;
-    for (uint32_t i = b0; i < b1; ++i) sum += v[i];
-    part[t] = sum;
+// Exclusive scan of m values in place, in blocks of 256: each block's sum,
+// an exclusive scan of the block sums (one workgroup, 256 at a time with a
+// running carry), then each block's own scan plus its offset.
+constexpr uint32_t kScanBlock = 256;
+
+// Inclusive sum over a 256-thread block (wave scans, wave totals through LDS).
+__device__ __forceinline__ uint32_t block_incl_sum(uint32_t v, uint32_t *wsum) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t x = wave_incl_sum(v);
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    for (uint32_t d = 1; d < kScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
-        const uint32_t add = t >= d ? part[t - d] : 0u;
+    uint32_t off = 0;
+    for (uint32_t q = 0; q < w; ++q) off += wsum[q];
+    __syncthreads();
+    return x + off;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ v,
+                                                            uint32_t m, uint32_t *__restrict__ part) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    const uint32_t x = block_incl_sum(i < m ? v[i] : 0u, wsum);
+    if (threadIdx.x == kScanBlock - 1) part[blockIdx.x] = x;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_parts(uint32_t *__restrict__ part, uint32_t g) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < g; b += kScanBlock) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t x = i < g ? part[i] : 0u;
+        const uint32_t inc = block_incl_sum(x, wsum);
+        const uint32_t base = carry;
+        if (i < g) part[i] = base + inc - x;
         __syncthreads();
-        part[t] += add;
+        if (threadIdx.x == kScanBlock - 1) carry = base + inc;
         __syncthreads();
     }
-    uint32_t run = part[t] - sum;
-    for (uint32_t i = b0; i < b1; ++i) {
-        const uint32_t x = v[i];
-        v[i] = run;
-        run += x;
-    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict__ v, uint32_t m,
+                                                           const uint32_t *__restrict__ part) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    const uint32_t x = i < m ? v[i] : 0u;
+    const uint32_t inc = block_incl_sum(x, wsum);
+    if (i < m) v[i] = part[blockIdx.x] + inc - x;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
@@ -2353,9 +2377,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
 }
 
+// Temporary space of the sort: the digit histograms, then the scan's block sums.
+static size_t sort_hist_words(uint32_t n) {
+    return std::max<size_t>(1, (n + kSortTile - 1) / kSortTile) * kRadix;
+}
+
 hipError_t sort_temp_bytes(uint32_t n, size_t *bytes) {
-    const size_t tiles = (n + kSortTile - 1) / kSortTile;
-    *bytes = std::max<size_t>(1, tiles) * kRadix * sizeof(uint32_t);
+    const size_t h = sort_hist_words(n);
+    *bytes = (h + (h + kScanBlock - 1) / kScanBlock) * sizeof(uint32_t);
     return hipSuccess;
 }
 
@@ -2365,14 +2394,20 @@ hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_a, uint32_t *key
     *perm = vals_a;
     if (p.n == 0) return hipSuccess;
     const uint32_t tiles = (p.n + kSortTile - 1) / kSortTile;
-    if (temp_bytes < (size_t)tiles * kRadix * sizeof(uint32_t)) return hipErrorInvalidValue;
+    size_t need = 0;
+    sort_temp_bytes(p.n, &need);
+    if (temp_bytes < need) return hipErrorInvalidValue;
+    const uint32_t m = tiles * kRadix, g = (m + kScanBlock - 1) / kScanBlock;
     uint32_t *hist = static_cast<uint32_t *>(temp);
+    uint32_t *part = hist + sort_hist_words(p.n);
     hipLaunchKernelGGL(k_sort_key, dim3((p.n + 255) / 256), dim3(256), 0, s, p, keys_a, vals_a);
     uint32_t *ki = keys_a, *ko = keys_b, *vi = vals_a, *vo = vals_b;
     for (uint32_t shift = 0; shift < 32; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, ki, p.n, shift,
                            tiles, hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kScanThreads), 0, s, hist, tiles * kRadix);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanBlock), 0, s, hist, m, part);
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanBlock), 0, s, part, g);
+        hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kScanBlock), 0, s, hist, m, part);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
                            p.n, shift, tiles, hist);
         std::swap(ki, ko);
