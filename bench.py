@@ -244,6 +244,20 @@ def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_r
         log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity), {len(live)} requests: latency "
             f"p50 {_pct(lat, 50):.3f} s, p99 {_pct(lat, 99):.3f} s; vs solo p50 "
             f"{_pct(vs, 50):.2f}x, p99 {_pct(vs, 99):.2f}x")
+    # the latency floor under load: the stream's smallest request submitted
+    # together with its k largest, against its solo time
+    small = min(reqs, key=lambda q: q["n"])
+    large = sorted(reqs, key=lambda q: -q["n"])[:3]
+    corun = []
+    for k in (1, 2, 3):
+        batch = [dict(q, arrival=0.0) for q in large[:k]] + [dict(small, arrival=0.0)]
+        o3, _ = B.run_server(cs, envs, batch, round_steps=round_steps)
+        lat = [o["latency"] for o in o3 if o["req"]["seed"] == small["seed"]][0]
+        corun.append({"large_requests": k, "large_spins": [q["n"] for q in large[:k]],
+                      "small_spins": small["n"], "small_latency_s": lat,
+                      "small_latency_vs_solo": lat / by_seed[small["seed"]]})
+        log(f"co-run: a {small['n']}-spin request with {k} of {[q['n'] for q in large[:k]]} spins: "
+            f"latency {lat:.3f} s, {lat / by_seed[small['seed']]:.2f}x solo")
     # the configured stream: far beyond one GPU's capacity, so a finite window
     n_over = int(overload_rate * overload_window * 1.5) + 10
     over = B.stream_requests(cs, envs, n_over, rate=overload_rate, seed=9)
@@ -287,7 +301,7 @@ def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_r
             # speedup cannot exceed 1 / this fraction
             "plain_spin_steps_per_s": B.work(reqs) / solo.sum(),
             "plain_rate_fraction_of_served": (B.work(reqs) / solo.sum()) / (B.work(reqs) / span),
-            "streams": streams,
+            "streams": streams, "small_request_with_large_ones": corun,
             "bit_identical_to_plain": bool(same), "configured_stream": overload}
 
 
