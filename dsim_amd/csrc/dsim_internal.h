@@ -17,6 +17,10 @@ constexpr uint32_t kFiberNone = 65535u;
 //                     vertex P1 (x,y,z) and info (fibre | membrane<<16)
 //   MT record:        n.x n.y n.z n.v0 | v0.x v0.y v0.z info | e1 - | e2 -
 // Labelling uses 32-byte AABB records: min.xyz info | max.xyz 0.
+// Spins per tile of the trajectory buffer (StepParams::traj): each tile's rows
+// are contiguous, so a phase GEMM spin tile reads one block.
+constexpr uint32_t kTrajTile = 128;
+
 struct StepParams {
     // substrate (device pointers)
     const float4 *tri_ref;
@@ -56,9 +60,10 @@ struct StepParams {
     uint32_t *wrap;   // periodic voxel crossings per spin (three signed 10-bit counts), used
                       // for the phase of the unwrapped position when moments are off
     uint32_t hit_window;  // steps those counts cover (0: no history)
-    float *traj;          // [3*iters][traj_stride] displacement after each step of the
-                          // launch, by spin id (moment mode), or null
-    uint32_t traj_stride;
+    float *traj;          // displacement after each step of the launch by spin id, in
+                          // tiles of kTrajTile spins: row r (3 * step + axis) of spin
+                          // s at [s / kTrajTile][r][s % kTrajTile] (moment mode), or null
+    uint32_t traj_stride;  // floats per tile: 3 * iters * kTrajTile
     // per call
     float gx, gy, gz, pc, weight;
     uint32_t key0, key1;
@@ -165,9 +170,11 @@ constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout k
 // Phase of arbitrary gradient waveforms (see kernels.hip, k_phase_gemm):
 // phi[s][w] += sum_k traj[k][s] * G[w][k] over one chunk of K = 3*steps,
 // with G given as bf16 hi and lo parts [n_wave][k_total] (k contiguous) and
-// phi[s][w] = (phase, Kahan compensation).  Spins padded to kPhaseTileM in
-// traj.
+// phi[s][w] = (phase, Kahan compensation).  traj in the tiled layout of
+// StepParams::traj with k_len rows (tiles of k_len * kTrajTile floats),
+// spins padded to kPhaseTileM.
 constexpr uint32_t kPhaseTileM = 128;  // spins per workgroup tile
+static_assert(kPhaseTileM == kTrajTile, "one trajectory tile per GEMM spin tile");
 constexpr uint32_t kPhaseTileN = 128;  // waveforms per workgroup tile
 constexpr uint32_t kPhaseSliceK = 32;  // k per LDS stage
 // k_len trajectory rows in Kahan folds of k_fold rows (both multiples of

@@ -1159,10 +1159,12 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
             }
             if (MOMENT && P.traj) {
                 // unwrapped displacement after this step, rows 3*it + axis
-                float *tr = P.traj + (size_t)(3 * it) * P.traj_stride + sid;
+                // of the spin's tile
+                float *tr = P.traj + (size_t)(sid / kTrajTile) * P.traj_stride +
+                            (3 * it) * kTrajTile + sid % kTrajTile;
                 tr[0] = dx;
-                tr[P.traj_stride] = dy;
-                tr[2 * (size_t)P.traj_stride] = dz;
+                tr[kTrajTile] = dy;
+                tr[2 * kTrajTile] = dz;
             }
             if (TRACE) {
                 uint32_t qx, qy, qz;
@@ -1383,143 +1385,175 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t kLdsPitch = 40;  // bf16 per LDS row: 32 k + 8 padding (80 bytes)
 
-__device__ __forceinline__ uint16_t bf16_bits_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Register stage of one K slice of the phase GEMM (see k_phase_gemm).
-struct PhaseStage {
-    float tv[16];
-    uint4 wh0, wh1, wl0, wl1;
+// Two floats to bf16 hi + lo parts, each packed two to a word (element 0 in
+// the low half): v_cvt_pk_bf16_f32 rounds to nearest even, and lo is the
+// rounded remainder x - hi, which is exact in FP32.
+__device__ __forceinline__ void bf16_split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+    const f32x2 v = {a, b};
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+    const f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+}
+
+// Raw buffer descriptor over a wave-uniform base (gfx9 word 3; no stride, byte
+// offsets below 2^31).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, 0x00020000);
+}
+
+// Register stage of one K slice of the phase GEMM (see k_phase_gemm) for
+// kPhaseThreads threads and a workgroup tile of 128 spins x 128 * TJ
+// waveforms: 8 consecutive k of one spin of the trajectory, and TJ 16-byte
+// pieces each of the hi and lo waveform rows.
+constexpr uint32_t kPhaseThreads = 512;
+template <int TJ> struct PhaseStage {
+    float tv[8];
+    u32x4 wh[TJ], wl[TJ];
 };
-__device__ __forceinline__ void phase_gload(PhaseStage &S, const float *__restrict__ traj,
-                                            uint32_t n_pad, uint32_t spin, uint32_t akg,
-                                            uint32_t ks, const uint16_t *__restrict__ g_hi,
+// Loads of a slice through buffer descriptors with wave-uniform bases: the
+// spin tile (a contiguous block of k_len rows of kTrajTile spins), the wave's
+// k row akg (t / 128 is uniform in a wave) and the waveform tile n0 go into
+// the scalar base, the row stride into the scalar offset, so the loads take
+// no vector address arithmetic.  The waveform offsets stay below 2^31 bytes:
+// 128 * TJ * k_total * 2 (checked at launch).
+template <int TJ>
+__device__ __forceinline__ void phase_gload(PhaseStage<TJ> &S, const float *__restrict__ traj,
+                                            uint32_t k_len, uint32_t tm, uint32_t am,
+                                            uint32_t akg, uint32_t ks,
+                                            const uint16_t *__restrict__ g_hi,
                                             const uint16_t *__restrict__ g_lo, uint32_t n0,
                                             uint32_t k_total, uint32_t k0, uint32_t t) {
-    const float *src = traj + (size_t)(ks + akg) * n_pad + spin;
+    const __amdgpu_buffer_rsrc_t ra =
+        raw_rsrc(traj + ((size_t)tm * k_len + ks + akg) * kTrajTile);
 #pragma unroll
-    for (uint32_t q = 0; q < 16; ++q) S.tv[q] = src[(size_t)q * n_pad];
-    // 16-byte piece c: row c/4, quarter c%4, for c = t and t + 256
-    const size_t g0 = (size_t)(n0 + (t >> 2)) * k_total + k0 + ks + (t & 3u) * 8u;
-    const size_t g1 = g0 + (size_t)64 * k_total;
-    S.wh0 = *reinterpret_cast<const uint4 *>(g_hi + g0);
-    S.wl0 = *reinterpret_cast<const uint4 *>(g_lo + g0);
-    S.wh1 = *reinterpret_cast<const uint4 *>(g_hi + g1);
-    S.wl1 = *reinterpret_cast<const uint4 *>(g_lo + g1);
-}
-__device__ __forceinline__ uint4 pack_bf16x8(const uint16_t *h) {
-    uint4 v;
-    v.x = h[0] | ((uint32_t)h[1] << 16);
-    v.y = h[2] | ((uint32_t)h[3] << 16);
-    v.z = h[4] | ((uint32_t)h[5] << 16);
-    v.w = h[6] | ((uint32_t)h[7] << 16);
-    return v;
-}
-// Trajectory split into bf16 hi + lo as [spin][k] rows (two 16-byte rows
-// each, distinct banks for 8 consecutive spins at the 80-byte pitch);
-// waveform rows as loaded.
-__device__ __forceinline__ void phase_lstore(const PhaseStage &S, uint16_t *a_hi, uint16_t *a_lo,
-                                             uint16_t *b_hi, uint16_t *b_lo, uint32_t am,
-                                             uint32_t akg, uint32_t t) {
-    uint16_t h[16], l[16];
+    for (uint32_t q = 0; q < 8; ++q)
+        S.tv[q] = __uint_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(ra, am * 4u, q * kTrajTile * 4u, 0));
+    const size_t gb = (size_t)n0 * k_total + k0 + ks;
+    const __amdgpu_buffer_rsrc_t rh = raw_rsrc(g_hi + gb), rl = raw_rsrc(g_lo + gb);
 #pragma unroll
-    for (uint32_t q = 0; q < 16; ++q) {
-        h[q] = bf16_bits_rne(S.tv[q]);
-        l[q] = bf16_bits_rne(S.tv[q] - bf16_to_float(h[q]));
+    for (int j = 0; j < TJ; ++j) {
+        // 16-byte piece c: waveform row c / 4, quarter c % 4
+        const uint32_t c = t + j * kPhaseThreads;
+        const uint32_t go = ((c >> 2) * k_total + (c & 3u) * 8u) * 2u;
+        S.wh[j] = __builtin_amdgcn_raw_buffer_load_b128(rh, go, 0, 0);
+        S.wl[j] = __builtin_amdgcn_raw_buffer_load_b128(rl, go, 0, 0);
     }
-    *reinterpret_cast<uint4 *>(&a_hi[am * kLdsPitch + akg]) = pack_bf16x8(h);
-    *reinterpret_cast<uint4 *>(&a_hi[am * kLdsPitch + akg + 8]) = pack_bf16x8(h + 8);
-    *reinterpret_cast<uint4 *>(&a_lo[am * kLdsPitch + akg]) = pack_bf16x8(l);
-    *reinterpret_cast<uint4 *>(&a_lo[am * kLdsPitch + akg + 8]) = pack_bf16x8(l + 8);
-    const uint32_t r0 = t >> 2, part = (t & 3u) * 8u;
-    *reinterpret_cast<uint4 *>(&b_hi[r0 * kLdsPitch + part]) = S.wh0;
-    *reinterpret_cast<uint4 *>(&b_lo[r0 * kLdsPitch + part]) = S.wl0;
-    *reinterpret_cast<uint4 *>(&b_hi[(r0 + 64) * kLdsPitch + part]) = S.wh1;
-    *reinterpret_cast<uint4 *>(&b_lo[(r0 + 64) * kLdsPitch + part]) = S.wl1;
+}
+// Trajectory split into bf16 hi + lo as [spin][k] rows (one 16-byte row piece
+// each); waveform rows as loaded.
+template <int TJ>
+__device__ __forceinline__ void phase_lstore(const PhaseStage<TJ> &S, uint16_t *a_hi,
+                                             uint16_t *a_lo, uint16_t *b_hi, uint16_t *b_lo,
+                                             uint32_t am, uint32_t akg, uint32_t t) {
+    uint32_t hq[4], lq[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) bf16_split2(S.tv[2 * q], S.tv[2 * q + 1], hq[q], lq[q]);
+    const u32x4 h = {hq[0], hq[1], hq[2], hq[3]}, l = {lq[0], lq[1], lq[2], lq[3]};
+    *reinterpret_cast<u32x4 *>(&a_hi[am * kLdsPitch + akg]) = h;
+    *reinterpret_cast<u32x4 *>(&a_lo[am * kLdsPitch + akg]) = l;
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+        const uint32_t c = t + j * kPhaseThreads, r0 = c >> 2, part = (c & 3u) * 8u;
+        *reinterpret_cast<u32x4 *>(&b_hi[r0 * kLdsPitch + part]) = S.wh[j];
+        *reinterpret_cast<u32x4 *>(&b_lo[r0 * kLdsPitch + part]) = S.wl[j];
+    }
 }
 
 // One launch covers k_len trajectory rows (the steps of one trajectory
 // buffer) in folds of k_fold rows: each fold's products are summed in the
 // MFMA accumulators from zero and then added, with Kahan compensation, to the
 // (phase, compensation) pairs, which stay in registers from the first fold
-// to the last and touch memory once per launch.  The K loop stages slices of
-// 32 through LDS, the next slice's global loads in flight while the current
-// slice's MFMAs run.
-__global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ traj, uint32_t n,
-                                                    uint32_t n_pad, uint32_t k_len,
-                                                    uint32_t k_fold,
-                                                    const uint16_t *__restrict__ g_hi,
-                                                    const uint16_t *__restrict__ g_lo,
-                                                    uint32_t k_total, uint32_t k0,
-                                                    uint32_t n_wave, float2 *__restrict__ phi) {
-    // two LDS stages: the MFMAs of slice s read one while slice s + 1 is split
-    // and stored into the other, one barrier per slice
-    constexpr uint32_t kOperandHalves = kPhaseTileM * kLdsPitch;
-    constexpr uint32_t kStage = 2 * kOperandHalves + 2 * kPhaseTileN * kLdsPitch;
+// to the last and touch memory once per launch.  The compensation is held
+// rounded to bf16, two to a register (it is below half an ulp of the phase,
+// so rounding it perturbs the sum by under 2^-10 ulp per fold); it is stored
+// as the float of that bf16, so a later launch reads it back exactly.
+// Workgroup tile 128 spins x 128 * TJ waveforms in eight waves of 64 x 32 * TJ,
+// two waves per SIMD.  The K loop stages slices of 32 through two LDS
+// stages: the next slice's global loads are in flight while the current
+// slice's MFMAs run, and its split and LDS stores follow, one barrier per
+// slice.
+template <int TJ>
+__global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
+    const float *__restrict__ traj, uint32_t n, uint32_t n_pad, uint32_t k_len, uint32_t k_fold,
+    const uint16_t *__restrict__ g_hi, const uint16_t *__restrict__ g_lo, uint32_t k_total,
+    uint32_t k0, uint32_t n_wave, float2 *__restrict__ phi) {
+    constexpr uint32_t kTileN = 128u * TJ;
+    constexpr uint32_t kAHalves = kPhaseTileM * kLdsPitch, kBHalves = kTileN * kLdsPitch;
+    constexpr uint32_t kStage = 2 * kAHalves + 2 * kBHalves;
     __shared__ __align__(16) uint16_t lds[2 * kStage];
-    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
     // Workgroups go to the 8 XCDs in turn; the numbering below hands all
     // waveform tiles of one spin tile to one XCD back to back, so the
     // trajectory slice is read from HBM once and then from that XCD's L2.
-    const uint32_t n_tiles = n_wave / kPhaseTileN, m_tiles = n_pad / kPhaseTileM;
+    const uint32_t n_tiles = n_wave / kTileN, m_tiles = n_pad / kPhaseTileM;
     const uint32_t xcd = blockIdx.x % 8u, r = blockIdx.x / 8u;
     const uint32_t tm = (r / n_tiles) * 8u + xcd;
     if (tm >= m_tiles) return;
-    const uint32_t n0 = (r % n_tiles) * kPhaseTileN;  // waveform tile
-    const uint32_t m0 = tm * kPhaseTileM;             // spin tile
-    const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 64u;  // wave tile in the WG tile
+    const uint32_t n0 = (r % n_tiles) * kTileN;  // waveform tile
+    const uint32_t m0 = tm * kPhaseTileM;        // spin tile
+    const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 32u * TJ;  // wave tile in the WG tile
     // v_mfma_f32_32x32x16_bf16 result register q of lane l: row
     // 8*(q/4) + 4*(l/32) + q%4, column l%32 of the 32 x 32 tile
     auto spin_of = [&](int i, int q) -> uint32_t {
         return m0 + wm + 32u * i + 8u * (q / 4) + 4u * (lane >> 5) + (q % 4);
     };
     auto wave_of = [&](int j) -> uint32_t { return n0 + wn + 32u * j + (lane & 31u); };
-    float ps[2][2][16], pk[2][2][16];  // phase and its Kahan compensation
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const uint32_t sp = spin_of(i, q);
-                const float2 v = sp < n ? phi[(size_t)sp * n_wave + wave_of(j)] : make_float2(0.f, 0.f);
-                ps[i][j][q] = v.x;
-                pk[i][j][q] = v.y;
-            }
-    // global -> register stage of a slice: the thread's spin (t % 128) and 16
-    // consecutive k of the trajectory; two 16-byte pieces of hi and of lo
-    // waveform rows
-    const uint32_t am = t & 127u, akg = (t >> 7) * 16u;
-    PhaseStage stg;
+    const uint32_t am = t & 127u, akg = (wave >> 1) * 8u;
+    auto stage = [&](uint32_t sl) -> uint16_t * { return lds + (sl & 1u) * kStage; };
+    auto store = [&](const PhaseStage<TJ> &S, uint16_t *st) {
+        phase_lstore<TJ>(S, st, st + kAHalves, st + 2 * kAHalves, st + 2 * kAHalves + kBHalves, am,
+                         akg, t);
+    };
+    PhaseStage<TJ> stg;
     const uint32_t n_slices = k_len / kPhaseSliceK, fold_slices = k_fold / kPhaseSliceK;
-    phase_gload(stg, traj, n_pad, m0 + am, akg, 0, g_hi, g_lo, n0, k_total, k0, t);
-    {
-        uint16_t *st = lds;
-        phase_lstore(stg, st, st + kOperandHalves, st + 2 * kOperandHalves,
-                     st + 2 * kOperandHalves + kPhaseTileN * kLdsPitch, am, akg, t);
-    }
-    phase_gload(stg, traj, n_pad, m0 + am, akg, min(1u, n_slices - 1) * kPhaseSliceK, g_hi, g_lo,
-                n0, k_total, k0, t);
-    __syncthreads();
-    f32x16 acc[2][2];
+    auto slice_k = [&](uint32_t sl) -> uint32_t { return min(sl, n_slices - 1) * kPhaseSliceK; };
+    // slice 0 into stage 0 and slice 1 into registers, then the pairs (issued
+    // after the slices' loads, so that waiting for those does not wait for
+    // the pairs)
+    phase_gload<TJ>(stg, traj, k_len, tm, am, akg, 0, g_hi, g_lo, n0, k_total, k0, t);
+    store(stg, stage(0));
+    phase_gload<TJ>(stg, traj, k_len, tm, am, akg, slice_k(1), g_hi, g_lo, n0, k_total, k0, t);
+    float ps[2][TJ][16];   // phase
+    uint32_t pk[2][TJ][8];  // compensation, bf16 of elements 2q (low half) and 2q + 1
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; q += 2) {
+                float c[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const uint32_t sp = spin_of(i, q + e);
+                    const float2 v = sp < n ? phi[(size_t)sp * n_wave + wave_of(j)] : make_float2(0.f, 0.f);
+                    ps[i][j][q + e] = v.x;
+                    c[e] = v.y;
+                }
+                const f32x2 cv = {c[0], c[1]};
+                pk[i][j][q / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(cv, bf16x2));
+            }
+    __syncthreads();
+    f32x16 acc[2][TJ];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
     for (uint32_t sl = 0; sl < n_slices; ++sl) {
-        const uint16_t *cur = lds + (sl & 1u) * kStage;
-        const uint16_t *a_hi = cur, *a_lo = cur + kOperandHalves;
-        const uint16_t *b_hi = cur + 2 * kOperandHalves, *b_lo = b_hi + kPhaseTileN * kLdsPitch;
+        const uint16_t *cur = stage(sl);
+        const uint16_t *a_hi = cur, *a_lo = cur + kAHalves;
+        const uint16_t *b_hi = cur + 2 * kAHalves, *b_lo = b_hi + kBHalves;
 #pragma unroll
         for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
             const uint32_t kofs = kk + 8u * (lane >> 5);
-            bf16x8 ah[2], al[2], bh[2], bl[2];
+            bf16x8 ah[2], al[2], bh[TJ], bl[TJ];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const uint32_t row = wm + 32u * i + (lane & 31u);
@@ -1527,7 +1561,7 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
                 al[i] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * kLdsPitch + kofs]);
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+            for (int j = 0; j < TJ; ++j) {
                 const uint32_t row = wn + 32u * j + (lane & 31u);
                 bh[j] = *reinterpret_cast<const bf16x8 *>(&b_hi[row * kLdsPitch + kofs]);
                 bl[j] = *reinterpret_cast<const bf16x8 *>(&b_lo[row * kLdsPitch + kofs]);
@@ -1535,17 +1569,17 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < TJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < TJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < TJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         }
         // the next slice into the other stage while these MFMAs run, and the
@@ -1553,26 +1587,29 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
         // can interleave the split with the MFMAs: after the last slice the
         // other stage is written and never read, and the loads stay on the
         // last slice)
-        {
-            uint16_t *nxt = lds + ((sl + 1) & 1u) * kStage;
-            phase_lstore(stg, nxt, nxt + kOperandHalves, nxt + 2 * kOperandHalves,
-                         nxt + 2 * kOperandHalves + kPhaseTileN * kLdsPitch, am, akg, t);
-            phase_gload(stg, traj, n_pad, m0 + am, akg,
-                        min(sl + 2, n_slices - 1) * kPhaseSliceK, g_hi, g_lo, n0, k_total, k0, t);
-        }
+        store(stg, stage(sl + 1));
+        phase_gload<TJ>(stg, traj, k_len, tm, am, akg, slice_k(sl + 2), g_hi, g_lo, n0, k_total,
+                        k0, t);
         if ((sl + 1) % fold_slices == 0 || sl + 1 == n_slices) {
             // Kahan fold of this chunk's sums
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < TJ; ++j)
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const float y = acc[i][j][q] - pk[i][j][q];
-                        const float s1 = ps[i][j][q] + y;
-                        pk[i][j][q] = (s1 - ps[i][j][q]) - y;
-                        ps[i][j][q] = s1;
-                        acc[i][j][q] = 0.0f;
+                    for (int q = 0; q < 16; q += 2) {
+                        float c[2] = {__uint_as_float(pk[i][j][q / 2] << 16),
+                                      __uint_as_float(pk[i][j][q / 2] & 0xffff0000u)};
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) {
+                            const float y = acc[i][j][q + e] - c[e];
+                            const float s1 = ps[i][j][q + e] + y;
+                            c[e] = (s1 - ps[i][j][q + e]) - y;
+                            ps[i][j][q + e] = s1;
+                            acc[i][j][q + e] = 0.0f;
+                        }
+                        const f32x2 cv = {c[0], c[1]};
+                        pk[i][j][q / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(cv, bf16x2));
                     }
         }
         __syncthreads();
@@ -1580,11 +1617,13 @@ __global__ __launch_bounds__(256) void k_phase_gemm(const float *__restrict__ tr
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const uint32_t sp = spin_of(i, q);
-                if (sp < n) phi[(size_t)sp * n_wave + wave_of(j)] = make_float2(ps[i][j][q], pk[i][j][q]);
+                const uint32_t w = pk[i][j][q / 2];
+                const float c = __uint_as_float((q & 1) ? (w & 0xffff0000u) : (w << 16));
+                if (sp < n) phi[(size_t)sp * n_wave + wave_of(j)] = make_float2(ps[i][j][q], c);
             }
 }
 
@@ -1593,11 +1632,22 @@ hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint
                              uint32_t k_total, uint32_t k0, uint32_t n_wave, float2 *phi,
                              hipStream_t s) {
     if (n == 0 || k_len == 0) return hipSuccess;
-    if (k_fold == 0 || k_fold % kPhaseSliceK || k_len % kPhaseSliceK) return hipErrorInvalidValue;
+    if (k_fold == 0 || k_fold % kPhaseSliceK || k_len % kPhaseSliceK || n_wave % kPhaseTileN ||
+        n_pad % kPhaseTileM)
+        return hipErrorInvalidValue;
+    // 256-waveform tiles where the waveforms divide into them
+    const bool wide = n_wave % (2 * kPhaseTileN) == 0;
+    const uint32_t tile_n = wide ? 2 * kPhaseTileN : kPhaseTileN;
+    // byte offsets of the waveform loads (phase_gload) below 2^31
+    if ((uint64_t)tile_n * k_total * 2 >= (1ull << 31)) return hipErrorInvalidValue;
     const uint32_t m_groups = (n_pad / kPhaseTileM + 7u) / 8u;
-    const uint32_t grid = m_groups * 8u * (n_wave / kPhaseTileN);
-    hipLaunchKernelGGL(k_phase_gemm, dim3(grid), dim3(256), 0, s, traj, n, n_pad, k_len, k_fold,
-                       g_hi, g_lo, k_total, k0, n_wave, phi);
+    const uint32_t grid = m_groups * 8u * (n_wave / tile_n);
+    if (wide)
+        hipLaunchKernelGGL(k_phase_gemm<2>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
+                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phi);
+    else
+        hipLaunchKernelGGL(k_phase_gemm<1>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
+                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phi);
     return hipGetLastError();
 }
 

@@ -25,7 +25,7 @@ struct dsim_waveforms {
     uint32_t buffer = 0;          // steps walked per trajectory buffer (a multiple of chunk)
     DevBuf<uint16_t> g_hi, g_lo;  // [n_wave][3*n_steps] bf16 bits, pc folded in
     DevBuf<double> gsum;          // [n_wave][3]: sum_t pc * G_w(t)
-    DevBuf<float> traj;           // [3*buffer][n_pad]
+    DevBuf<float> traj;           // n_pad / kTrajTile tiles of [3*len][kTrajTile] (len <= buffer)
     DevBuf<float2> phi;           // [n][n_wave]: (phase, Kahan compensation)
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
@@ -139,7 +139,7 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
     for (uint32_t c = 0; c < steps && rc == DSIM_OK;) {
         const uint32_t len = std::min(W->buffer, steps - c);  // a multiple of 32
         HIP_TRY(hipEventRecord(ev[0], st));
-        if ((rc = dsim_host::sim_walk_traj(W->sim, len, W->traj.p, W->n_pad))) break;
+        if ((rc = dsim_host::sim_walk_traj(W->sim, len, W->traj.p))) break;
         HIP_TRY(hipEventRecord(ev[1], st));
         hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len, 3 * W->chunk,
                                                W->g_hi.p, W->g_lo.p, 3 * W->n_steps,
@@ -150,10 +150,20 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
         }
         HIP_TRY(hipEventRecord(ev[2], st));
         if (traj_out) {
-            for (uint32_t r = 0; r < 3 * len; ++r)
-                HIP_TRY(hipMemcpyAsync(traj_out + ((size_t)3 * c + r) * W->n,
-                                       W->traj.p + (size_t)r * W->n_pad, (size_t)W->n * 4,
-                                       hipMemcpyDeviceToHost, st));
+            // rows [3 * len][n] out of the spin tiles
+            const size_t tile = (size_t)3 * len * dsim::kTrajTile;  // floats per tile
+            const uint32_t full = W->n / dsim::kTrajTile, rest = W->n % dsim::kTrajTile;
+            for (uint32_t r = 0; r < 3 * len; ++r) {
+                float *dst = traj_out + ((size_t)3 * c + r) * W->n;
+                const float *src = W->traj.p + (size_t)r * dsim::kTrajTile;
+                if (full)
+                    HIP_TRY(hipMemcpy2DAsync(dst, dsim::kTrajTile * 4, src, tile * 4,
+                                             dsim::kTrajTile * 4, full, hipMemcpyDeviceToHost, st));
+                if (rest)
+                    HIP_TRY(hipMemcpyAsync(dst + (size_t)full * dsim::kTrajTile,
+                                           src + full * tile, (size_t)rest * 4,
+                                           hipMemcpyDeviceToHost, st));
+            }
         }
         HIP_TRY(hipEventSynchronize(ev[2]));
         float a = 0.f, b = 0.f;

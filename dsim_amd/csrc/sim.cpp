@@ -775,7 +775,7 @@ uint64_t dsim_host::sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_l
     return b;
 }
 
-int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj, uint32_t traj_stride) {
+int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj) {
     if (int rc = check_sim(s)) return rc;
     if (s->prm.mode != DSIM_MODE_PHYSICAL || s->sorting())
         return fail(DSIM_ERR_STATE, "waveform runs need the physical mode without reordering");
@@ -783,7 +783,7 @@ int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj, uint32_t 
     dsim::StepParams P = step_params(s);
     P.weight = 0.0f;
     P.traj = traj;
-    P.traj_stride = traj_stride;
+    P.traj_stride = 3 * iters * dsim::kTrajTile;
     // one launch for the whole chunk: the kernel writes rows 3*it..3*it+2
     const uint32_t spl = s->prm.steps_per_launch;
     s->prm.steps_per_launch = iters;

@@ -20,10 +20,11 @@ int sim_batch_params(dsim_sim *s, float w, dsim::StepParams *P);
 void sim_batch_advance(dsim_sim *s, float w, uint32_t iters);
 hipStream_t sim_stream(const dsim_sim *s);
 // Advances `s` by `iters` steps (moment mode, weight 0: no moment change)
-// writing the unwrapped displacement after every step to traj
-// [3*iters][traj_stride] by spin id.  Requires dsim_moment_reset and no
-// reordering (the phase readout pairs traj rows with state slots).
-int sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj, uint32_t traj_stride);
+// writing the unwrapped displacement after every step to traj by spin id, in
+// tiles of kTrajTile spins of 3 * iters rows each ([s / kTrajTile][3 * step +
+// axis][s % kTrajTile]).  Requires dsim_moment_reset and no reordering (the
+// phase readout pairs traj rows with state slots).
+int sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj);
 // Device pointers of the state fields the waveform readout needs.
 void sim_readout_fields(dsim_sim *s, const float **mag, const uint32_t **label, const float **x0);
 uint32_t sim_spins(const dsim_sim *s);
