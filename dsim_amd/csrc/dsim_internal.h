@@ -169,8 +169,10 @@ constexpr uint32_t kSignalBlocks = 1024;  // partial-sum blocks of the readout k
 
 // Phase of arbitrary gradient waveforms (see kernels.hip, k_phase_gemm):
 // phi[s][w] += sum_k traj[k][s] * G[w][k] over one chunk of K = 3*steps,
-// with G given as bf16 hi and lo parts [n_wave][k_total] (k contiguous) and
-// phi[s][w] = (phase, Kahan compensation).  traj in the tiled layout of
+// with G given as bf16 hi and lo parts [n_wave][k_total] (k contiguous),
+// phase[s][w] in FP32 ([n_pad][n_wave]) and its Kahan compensation as bf16,
+// spins 2m and 2m + 1 in the low and high half of comp[m][w]
+// ([n_pad / 2][n_wave]).  traj in the tiled layout of
 // StepParams::traj with k_len rows (tiles of k_len * kTrajTile floats),
 // spins padded to kPhaseTileM.
 constexpr uint32_t kPhaseTileM = 128;  // spins per workgroup tile
@@ -181,13 +183,13 @@ constexpr uint32_t kPhaseSliceK = 32;  // k per LDS stage
 // kPhaseSliceK; the last fold may be shorter).
 hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_len,
                              uint32_t k_fold, const uint16_t *g_hi, const uint16_t *g_lo,
-                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float2 *phi,
-                             hipStream_t s);
+                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float *phase,
+                             uint32_t *comp, hipStream_t s);
 // Signal of every waveform: per spin block and waveform the FP64 sums of
 // m cos(phase), m sin(phase) and counts for the total and compartments 0-3,
-// phase = phi[s][w] + gsum[w] . x0[s] (in FP64); then a fixed-order final
-// pass into out[n_wave][15].
-hipError_t launch_phase_signal(uint32_t n, const float2 *phi, uint32_t n_wave,
+// phase[s][w] + gsum[w] . x0[s] (in FP64); then a fixed-order final pass
+// into out[n_wave][15].
+hipError_t launch_phase_signal(uint32_t n, const float *phase, uint32_t n_wave,
                                const double *gsum, const float *mag, const uint32_t *label,
                                const float *x0, double *partial, uint32_t n_blocks,
                                double *out, hipStream_t s);

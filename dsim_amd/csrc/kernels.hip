@@ -1469,9 +1469,10 @@ __device__ __forceinline__ void phase_lstore(const PhaseStage<TJ> &S, uint16_t *
 // MFMA accumulators from zero and then added, with Kahan compensation, to the
 // (phase, compensation) pairs, which stay in registers from the first fold
 // to the last and touch memory once per launch.  The compensation is held
-// rounded to bf16, two to a register (it is below half an ulp of the phase,
-// so rounding it perturbs the sum by under 2^-10 ulp per fold); it is stored
-// as the float of that bf16, so a later launch reads it back exactly.
+// rounded to bf16 (it is below half an ulp of the phase, so rounding it
+// perturbs the sum by under 2^-10 ulp per fold), two to a word: the spins
+// 2m and 2m + 1 of one waveform, which are elements q and q + 1 (q even) of
+// one MFMA result tile, so each word loads and stores as it is held.
 // Workgroup tile 128 spins x 128 * TJ waveforms in eight waves of 64 x 32 * TJ,
 // two waves per SIMD.  The K loop stages slices of 32 through two LDS
 // stages: the next slice's global loads are in flight while the current
@@ -1481,7 +1482,7 @@ template <int TJ>
 __global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
     const float *__restrict__ traj, uint32_t n, uint32_t n_pad, uint32_t k_len, uint32_t k_fold,
     const uint16_t *__restrict__ g_hi, const uint16_t *__restrict__ g_lo, uint32_t k_total,
-    uint32_t k0, uint32_t n_wave, float2 *__restrict__ phi) {
+    uint32_t k0, uint32_t n_wave, float *__restrict__ phase, uint32_t *__restrict__ comp) {
     constexpr uint32_t kTileN = 128u * TJ;
     constexpr uint32_t kAHalves = kPhaseTileM * kLdsPitch, kBHalves = kTileN * kLdsPitch;
     constexpr uint32_t kStage = 2 * kAHalves + 2 * kBHalves;
@@ -1500,10 +1501,6 @@ __global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
     const uint32_t wm = (wave & 1u) * 64u, wn = (wave >> 1) * 32u * TJ;  // wave tile in the WG tile
     // v_mfma_f32_32x32x16_bf16 result register q of lane l: row
     // 8*(q/4) + 4*(l/32) + q%4, column l%32 of the 32 x 32 tile
-    auto spin_of = [&](int i, int q) -> uint32_t {
-        return m0 + wm + 32u * i + 8u * (q / 4) + 4u * (lane >> 5) + (q % 4);
-    };
-    auto wave_of = [&](int j) -> uint32_t { return n0 + wn + 32u * j + (lane & 31u); };
     const uint32_t am = t & 127u, akg = (wave >> 1) * 8u;
     auto stage = [&](uint32_t sl) -> uint16_t * { return lds + (sl & 1u) * kStage; };
     auto store = [&](const PhaseStage<TJ> &S, uint16_t *st) {
@@ -1520,24 +1517,35 @@ __global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
     store(stg, stage(0));
     phase_gload<TJ>(stg, traj, k_len, tm, am, akg, slice_k(1), g_hi, g_lo, n0, k_total, k0, t);
     float ps[2][TJ][16];   // phase
-    uint32_t pk[2][TJ][8];  // compensation, bf16 of elements 2q (low half) and 2q + 1
+    uint32_t pk[2][TJ][8];  // compensation, bf16 of elements q (low half) and q + 1, q even
+    // Pairs through buffer descriptors based at the wave tile's first spin
+    // and waveform (wave-uniform), so no per-element addresses live across
+    // the K loop: element (i, j, q) is row 32 i + 8 (q / 4) + 4 (l / 32) +
+    // q % 4 of the wave tile, column 32 j + l % 32; the lane's row and column
+    // and q % 4 go into four lane offsets, 32 i + 8 (q / 4) into the scalar
+    // offset and 32 j into the immediate one.  phase has n_pad rows.
+    const __amdgpu_buffer_rsrc_t rp = raw_rsrc(phase + (size_t)(m0 + wm) * n_wave + n0 + wn);
+    const __amdgpu_buffer_rsrc_t rc = raw_rsrc(comp + (size_t)((m0 + wm) / 2) * n_wave + n0 + wn);
+    uint32_t lp[4], lc[2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lp[r] = ((4u * (lane >> 5) + r) * n_wave + (lane & 31u)) * 4u;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) lc[r] = ((2u * (lane >> 5) + r) * n_wave + (lane & 31u)) * 4u;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < TJ; ++j)
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t sp = (32u * i + 8u * (q / 4)) * n_wave * 4u;
+            const uint32_t sc = (16u * i + 4u * (q / 4)) * n_wave * 4u;
 #pragma unroll
-            for (int q = 0; q < 16; q += 2) {
-                float c[2];
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const uint32_t sp = spin_of(i, q + e);
-                    const float2 v = sp < n ? phi[(size_t)sp * n_wave + wave_of(j)] : make_float2(0.f, 0.f);
-                    ps[i][j][q + e] = v.x;
-                    c[e] = v.y;
-                }
-                const f32x2 cv = {c[0], c[1]};
-                pk[i][j][q / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(cv, bf16x2));
+            for (int j = 0; j < TJ; ++j) {
+                ps[i][j][q] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(rp, lp[q % 4] + 128u * j, sp, 0));
+                if (q % 2 == 0)
+                    pk[i][j][q / 2] =
+                        __builtin_amdgcn_raw_buffer_load_b32(rc, lc[(q % 4) / 2] + 128u * j, sc, 0);
             }
+        }
     __syncthreads();
     f32x16 acc[2][TJ];
 #pragma unroll
@@ -1546,91 +1554,98 @@ __global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
         for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
-    for (uint32_t sl = 0; sl < n_slices; ++sl) {
-        const uint16_t *cur = stage(sl);
-        const uint16_t *a_hi = cur, *a_lo = cur + kAHalves;
-        const uint16_t *b_hi = cur + 2 * kAHalves, *b_lo = b_hi + kBHalves;
+    // K in folds of fold_slices slices: the slices of a fold, then its
+    // Kahan fold (one loop shape, so the pairs stay in registers throughout)
+    for (uint32_t f0 = 0; f0 < n_slices; f0 += fold_slices) {
+        const uint32_t f1 = min(f0 + fold_slices, n_slices);
+        for (uint32_t sl = f0; sl < f1; ++sl) {
+            const uint16_t *cur = stage(sl);
+            const uint16_t *a_hi = cur, *a_lo = cur + kAHalves;
+            const uint16_t *b_hi = cur + 2 * kAHalves, *b_lo = b_hi + kBHalves;
 #pragma unroll
-        for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
-            const uint32_t kofs = kk + 8u * (lane >> 5);
-            bf16x8 ah[2], al[2], bh[TJ], bl[TJ];
+            for (uint32_t kk = 0; kk < kPhaseSliceK; kk += 16) {
+                const uint32_t kofs = kk + 8u * (lane >> 5);
+                bf16x8 ah[2], al[2], bh[TJ], bl[TJ];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const uint32_t row = wm + 32u * i + (lane & 31u);
-                ah[i] = *reinterpret_cast<const bf16x8 *>(&a_hi[row * kLdsPitch + kofs]);
-                al[i] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * kLdsPitch + kofs]);
+                for (int i = 0; i < 2; ++i) {
+                    const uint32_t row = wm + 32u * i + (lane & 31u);
+                    ah[i] = *reinterpret_cast<const bf16x8 *>(&a_hi[row * kLdsPitch + kofs]);
+                    al[i] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * kLdsPitch + kofs]);
+                }
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    const uint32_t row = wn + 32u * j + (lane & 31u);
+                    bh[j] = *reinterpret_cast<const bf16x8 *>(&b_hi[row * kLdsPitch + kofs]);
+                    bl[j] = *reinterpret_cast<const bf16x8 *>(&b_lo[row * kLdsPitch + kofs]);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
             }
-#pragma unroll
-            for (int j = 0; j < TJ; ++j) {
-                const uint32_t row = wn + 32u * j + (lane & 31u);
-                bh[j] = *reinterpret_cast<const bf16x8 *>(&b_hi[row * kLdsPitch + kofs]);
-                bl[j] = *reinterpret_cast<const bf16x8 *>(&b_lo[row * kLdsPitch + kofs]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            // the next slice into the other stage while these MFMAs run, and the
+            // one after it into registers (unconditionally, so that the compiler
+            // can interleave the split with the MFMAs: after the last slice the
+            // other stage is written and never read, and the loads stay on the
+            // last slice)
+            store(stg, stage(sl + 1));
+            phase_gload<TJ>(stg, traj, k_len, tm, am, akg, slice_k(sl + 2), g_hi, g_lo, n0, k_total,
+                            k0, t);
+            __syncthreads();
         }
-        // the next slice into the other stage while these MFMAs run, and the
-        // one after it into registers (unconditionally, so that the compiler
-        // can interleave the split with the MFMAs: after the last slice the
-        // other stage is written and never read, and the loads stay on the
-        // last slice)
-        store(stg, stage(sl + 1));
-        phase_gload<TJ>(stg, traj, k_len, tm, am, akg, slice_k(sl + 2), g_hi, g_lo, n0, k_total,
-                        k0, t);
-        if ((sl + 1) % fold_slices == 0 || sl + 1 == n_slices) {
-            // Kahan fold of this chunk's sums
+        // Kahan fold of this chunk's sums
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < TJ; ++j)
+            for (int j = 0; j < TJ; ++j)
 #pragma unroll
-                    for (int q = 0; q < 16; q += 2) {
-                        float c[2] = {__uint_as_float(pk[i][j][q / 2] << 16),
-                                      __uint_as_float(pk[i][j][q / 2] & 0xffff0000u)};
+                for (int q = 0; q < 16; q += 2) {
+                    float c[2] = {__uint_as_float(pk[i][j][q / 2] << 16),
+                                  __uint_as_float(pk[i][j][q / 2] & 0xffff0000u)};
 #pragma unroll
-                        for (int e = 0; e < 2; ++e) {
-                            const float y = acc[i][j][q + e] - c[e];
-                            const float s1 = ps[i][j][q + e] + y;
-                            c[e] = (s1 - ps[i][j][q + e]) - y;
-                            ps[i][j][q + e] = s1;
-                            acc[i][j][q + e] = 0.0f;
-                        }
-                        const f32x2 cv = {c[0], c[1]};
-                        pk[i][j][q / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(cv, bf16x2));
+                    for (int e = 0; e < 2; ++e) {
+                        const float y = acc[i][j][q + e] - c[e];
+                        const float s1 = ps[i][j][q + e] + y;
+                        c[e] = (s1 - ps[i][j][q + e]) - y;
+                        ps[i][j][q + e] = s1;
+                        acc[i][j][q + e] = 0.0f;
                     }
-        }
-        __syncthreads();
+                    const f32x2 cv = {c[0], c[1]};
+                    pk[i][j][q / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(cv, bf16x2));
+                }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < TJ; ++j)
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t sp = (32u * i + 8u * (q / 4)) * n_wave * 4u;
+            const uint32_t sc = (16u * i + 4u * (q / 4)) * n_wave * 4u;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const uint32_t sp = spin_of(i, q);
-                const uint32_t w = pk[i][j][q / 2];
-                const float c = __uint_as_float((q & 1) ? (w & 0xffff0000u) : (w << 16));
-                if (sp < n) phi[(size_t)sp * n_wave + wave_of(j)] = make_float2(ps[i][j][q], c);
+            for (int j = 0; j < TJ; ++j) {
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ps[i][j][q]), rp,
+                                                      lp[q % 4] + 128u * j, sp, 0);
+                if (q % 2 == 0)
+                    __builtin_amdgcn_raw_buffer_store_b32(pk[i][j][q / 2], rc,
+                                                          lc[(q % 4) / 2] + 128u * j, sc, 0);
             }
+        }
 }
 
 hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint32_t k_len,
                              uint32_t k_fold, const uint16_t *g_hi, const uint16_t *g_lo,
-                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float2 *phi,
-                             hipStream_t s) {
+                             uint32_t k_total, uint32_t k0, uint32_t n_wave, float *phase,
+                             uint32_t *comp, hipStream_t s) {
     if (n == 0 || k_len == 0) return hipSuccess;
     if (k_fold == 0 || k_fold % kPhaseSliceK || k_len % kPhaseSliceK || n_wave % kPhaseTileN ||
         n_pad % kPhaseTileM)
@@ -1638,22 +1653,23 @@ hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint
     // 256-waveform tiles where the waveforms divide into them
     const bool wide = n_wave % (2 * kPhaseTileN) == 0;
     const uint32_t tile_n = wide ? 2 * kPhaseTileN : kPhaseTileN;
-    // byte offsets of the waveform loads (phase_gload) below 2^31
-    if ((uint64_t)tile_n * k_total * 2 >= (1ull << 31)) return hipErrorInvalidValue;
+    // byte offsets of the waveform loads (phase_gload) and of the pairs below 2^31
+    if ((uint64_t)tile_n * k_total * 2 >= (1ull << 31) || (uint64_t)64 * n_wave * 4 >= (1ull << 31))
+        return hipErrorInvalidValue;
     const uint32_t m_groups = (n_pad / kPhaseTileM + 7u) / 8u;
     const uint32_t grid = m_groups * 8u * (n_wave / tile_n);
     if (wide)
         hipLaunchKernelGGL(k_phase_gemm<2>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
-                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phi);
+                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phase, comp);
     else
         hipLaunchKernelGGL(k_phase_gemm<1>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
-                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phi);
+                           k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phase, comp);
     return hipGetLastError();
 }
 
 // Readout of the waveform phases: one thread per waveform, one block row per
 // range of spins; FP64 phase (x0 term added in FP64), cos/sin and sums.
-__global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float2 *__restrict__ phi,
+__global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *__restrict__ phase,
                                                        uint32_t n_wave,
                                                        const double *__restrict__ gsum,
                                                        const float *__restrict__ mag,
@@ -1668,7 +1684,7 @@ __global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float2 
     for (int k = 0; k < 15; ++k) v[k] = 0.0;
     const uint32_t s1 = min(n, (sb + 1) * per_block);
     for (uint32_t s = sb * per_block; s < s1; ++s) {
-        const double ph = (double)phi[(size_t)s * n_wave + w].x + gx * (double)x0[s] +
+        const double ph = (double)phase[(size_t)s * n_wave + w] + gx * (double)x0[s] +
                           gy * (double)x0[n + s] + gz * (double)x0[2 * n + s];
         double sn, cs;
         sincos(ph, &sn, &cs);
@@ -1690,14 +1706,14 @@ __global__ __launch_bounds__(256) void k_phase_final(const double *partial, uint
     out[i] = acc;
 }
 
-hipError_t launch_phase_signal(uint32_t n, const float2 *phi, uint32_t n_wave,
+hipError_t launch_phase_signal(uint32_t n, const float *phase, uint32_t n_wave,
                                const double *gsum, const float *mag, const uint32_t *label,
                                const float *x0, double *partial, uint32_t n_blocks,
                                double *out, hipStream_t s) {
     if (n_wave == 0 || n_blocks == 0) return hipSuccess;
     const uint32_t per = (n + n_blocks - 1) / n_blocks;
     hipLaunchKernelGGL(k_phase_partial, dim3((n_wave + 255) / 256, n_blocks), dim3(256), 0, s, n,
-                       phi, n_wave, gsum, mag, label, x0, per, partial);
+                       phase, n_wave, gsum, mag, label, x0, per, partial);
     hipLaunchKernelGGL(k_phase_final, dim3((n_wave * 15 + 255) / 256), dim3(256), 0, s, partial,
                        n_blocks, n_wave, out);
     return hipGetLastError();

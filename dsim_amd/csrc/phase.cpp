@@ -26,11 +26,14 @@ struct dsim_waveforms {
     DevBuf<uint16_t> g_hi, g_lo;  // [n_wave][3*n_steps] bf16 bits, pc folded in
     DevBuf<double> gsum;          // [n_wave][3]: sum_t pc * G_w(t)
     DevBuf<float> traj;           // n_pad / kTrajTile tiles of [3*len][kTrajTile] (len <= buffer)
-    DevBuf<float2> phi;           // [n][n_wave]: (phase, Kahan compensation)
+    DevBuf<float> phase;          // [n_pad][n_wave] (rows past n: padding)
+    DevBuf<uint32_t> comp;        // [n_pad / 2][n_wave]: Kahan compensation of spins 2m, 2m + 1
+                                  // as bf16 in the low and high half
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
     ~dsim_waveforms() {
-        g_hi.release(); g_lo.release(); gsum.release(); traj.release(); phi.release();
+        g_hi.release(); g_lo.release(); gsum.release(); traj.release(); phase.release();
+        comp.release();
         partial.release(); out.release();
     }
 };
@@ -86,9 +89,10 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
                 gs[(size_t)w * 3 + a] += (double)v;
             }
     int rc;
-    const size_t nw = (size_t)std::max<uint32_t>(W->n, 1) * n_wave;
+    const size_t nw = (size_t)std::max<uint32_t>(W->n_pad, 1) * n_wave;
     if ((rc = upload(W->g_hi, hi)) || (rc = upload(W->g_lo, lo)) || (rc = upload(W->gsum, gs)) ||
-        (rc = W->traj.alloc((size_t)3 * chunk * W->n_pad)) || (rc = W->phi.alloc(nw)) ||
+        (rc = W->traj.alloc((size_t)3 * chunk * W->n_pad)) || (rc = W->phase.alloc(nw)) ||
+        (rc = W->comp.alloc((size_t)std::max<uint32_t>(W->n_pad / 2, 1) * n_wave)) ||
         (rc = W->partial.alloc((size_t)kPhaseBlocks * n_wave * 15)) ||
         (rc = W->out.alloc((size_t)n_wave * 15))) {
         delete W;
@@ -96,7 +100,8 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     }
     // padded trajectory columns stay zero; phases start at zero
     hipError_t e = hipMemset(W->traj.p, 0, W->traj.n * 4);
-    if (e == hipSuccess) e = hipMemset(W->phi.p, 0, nw * 8);
+    if (e == hipSuccess) e = hipMemset(W->phase.p, 0, nw * 4);
+    if (e == hipSuccess) e = hipMemset(W->comp.p, 0, W->comp.n * 4);
     if (e != hipSuccess) {
         delete W;
         return fail(DSIM_ERR_HIP, "waveform init: %s", hipGetErrorString(e));
@@ -143,7 +148,8 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
         HIP_TRY(hipEventRecord(ev[1], st));
         hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len, 3 * W->chunk,
                                                W->g_hi.p, W->g_lo.p, 3 * W->n_steps,
-                                               3 * W->done, W->n_wave, W->phi.p, st);
+                                               3 * W->done, W->n_wave, W->phase.p, W->comp.p,
+                                               st);
         if (e != hipSuccess) {
             rc = fail(DSIM_ERR_HIP, "phase GEMM: %s", hipGetErrorString(e));
             break;
@@ -186,7 +192,7 @@ extern "C" int dsim_waveforms_signal(dsim_waveforms *W, double *out) {
     dsim_host::sim_readout_fields(W->sim, &mag, &label, &x0);
     hipStream_t st = dsim_host::sim_stream(W->sim);
     const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(kPhaseBlocks, W->n));
-    hipError_t e = dsim::launch_phase_signal(W->n, W->phi.p, W->n_wave, W->gsum.p, mag, label, x0,
+    hipError_t e = dsim::launch_phase_signal(W->n, W->phase.p, W->n_wave, W->gsum.p, mag, label, x0,
                                              W->partial.p, nb, W->out.p, st);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "phase readout: %s", hipGetErrorString(e));
     std::vector<double> v((size_t)W->n_wave * 15);
@@ -205,11 +211,16 @@ extern "C" int dsim_waveforms_get_phase(dsim_waveforms *W, float *phase, float *
     if (!W) return fail(DSIM_ERR_ARG, "NULL waveforms");
     if (int rc = dsim_sync(W->sim)) return rc;
     const size_t m = (size_t)W->n * W->n_wave;
-    std::vector<float2> v(m);
-    if (m) HIP_TRY(hipMemcpy(v.data(), W->phi.p, m * 8, hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < m; ++k) {
-        if (phase) phase[k] = v[k].x;
-        if (comp) comp[k] = v[k].y;
+    if (phase && m) HIP_TRY(hipMemcpy(phase, W->phase.p, m * 4, hipMemcpyDeviceToHost));
+    if (comp && m) {
+        std::vector<uint32_t> v((size_t)(W->n + 1) / 2 * W->n_wave);
+        HIP_TRY(hipMemcpy(v.data(), W->comp.p, v.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t s = 0; s < W->n; ++s)
+            for (uint32_t w = 0; w < W->n_wave; ++w) {
+                const uint32_t word = v[(size_t)(s / 2) * W->n_wave + w];
+                const uint32_t bits = (s & 1u) ? (word & 0xffff0000u) : (word << 16);
+                std::memcpy(&comp[(size_t)s * W->n_wave + w], &bits, 4);
+            }
     }
     return DSIM_OK;
 }
