@@ -445,7 +445,9 @@ def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=1024):
     sim.close()
     flops = 3 * 2.0 * n_spins * n_wave * 3 * steps   # hi*hi + hi*lo + lo*hi
     n_buffers = -(-steps // buffer)
-    gemm_bytes = n_buffers * (16.0 * n_spins * n_wave) + 4.0 * n_spins * 3 * steps
+    # per buffer the (phase, compensation) pairs are read and written once (4 B
+    # phase + 2 B bf16 compensation each way); the trajectory is read once
+    gemm_bytes = n_buffers * (12.0 * n_spins * n_wave) + 4.0 * n_spins * 3 * steps
     log(f"waveforms: {n_spins} spins x {steps} steps x {n_wave} waveforms in {wall:.2f} s "
         f"(walk {walk_ms / 1e3:.2f} s, phase GEMM {gemm_ms / 1e3:.2f} s = "
         f"{flops / (gemm_ms / 1e3) / 1e12:.0f} TFLOP/s); b=0-free signal range "
@@ -484,7 +486,7 @@ def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=1024):
             "phase_gemm_s": gemm_ms / 1e3,
             # algorithmic flops: the three bf16 products (hi*hi, hi*lo, lo*hi)
             # of the spins x (3 * steps) x waveforms GEMM; bytes: the
-            # read-modify-write of the (phase, compensation) pairs, 8 B per
+            # read-modify-write of the (phase, compensation) pairs, 6 B per
             # spin and waveform each way, once per trajectory buffer, plus one
             # read of the trajectory (4 B per spin and k)
             "phase_gemm_roofline": {"bound": "mfma", "achieved": flops / (gemm_ms / 1e3) / 1e12,

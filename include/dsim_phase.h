@@ -9,8 +9,8 @@
  * every waveform: the unwrapped displacement of every spin is recorded chunk by
  * chunk and multiplied on the matrix cores by the waveforms' gradients, the
  * phase of each (spin, waveform) pair kept in FP32 with Kahan compensation
- * across chunks (hi/lo bf16 split products, FP32 accumulation), and the signals
- * are summed in FP64.
+ * across chunks (hi/lo bf16 split products, FP32 accumulation; the compensation
+ * held as bf16), and the signals are summed in FP64.
  *
  * The handle must use the physical mode without spin reordering
  * (sort_interval 0), and dsim_moment_reset must have been called: the phase is
@@ -35,10 +35,10 @@ typedef struct dsim_waveforms dsim_waveforms;
  * 3 * chunk_steps) are summed in FP32 and then added to the phase with
  * compensation; the last chunk of a run may be shorter.  grads: host array
  * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
- * phase_constant = 2*pi*gamma*dt.  Device memory: 8 bytes per (spin,
- * waveform) for the phase and its compensation, plus 4 * n_wave * 3 * n_steps
- * bytes of bf16 gradients and one trajectory buffer (12 bytes per spin and
- * buffered step). */
+ * phase_constant = 2*pi*gamma*dt.  Device memory: 6 bytes per (spin,
+ * waveform) for the phase (FP32) and its compensation (bf16), spins padded to
+ * a multiple of 128, plus 4 * n_wave * 3 * n_steps bytes of bf16 gradients and
+ * one trajectory buffer (12 bytes per spin and buffered step). */
 int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_steps, const float *grads,
                           float phase_constant, uint32_t chunk_steps, dsim_waveforms **out);
 int dsim_waveforms_destroy(dsim_waveforms *wf);
@@ -56,7 +56,7 @@ int dsim_waveforms_run(dsim_waveforms *wf, uint32_t steps, float *traj_out);
  * getMrSignal (spinSystem.cpp:1452-1500), from the FP64 sums. */
 int dsim_waveforms_signal(dsim_waveforms *wf, double *out);
 /* Host copy of the accumulated phases without the x0 term, [n_spins][n_wave],
- * and their Kahan compensations (either may be NULL). */
+ * and their Kahan compensations (values of bf16; either may be NULL). */
 int dsim_waveforms_get_phase(dsim_waveforms *wf, float *phase, float *comp);
 /* Milliseconds spent in the walk and in the phase GEMM since creation. */
 int dsim_waveforms_timing(dsim_waveforms *wf, double *walk_ms, double *gemm_ms);
