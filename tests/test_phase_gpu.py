@@ -48,8 +48,11 @@ def _fp64_signal(phase, mag, comp):
     return out
 
 
-def test_phase_gemm_matches_fp64(env):
-    n, n_wave, steps, chunk = 3000, 256, 256, 64
+# 256 waveforms take the 128 x 256 workgroup tile, 384 (three tiles of 128) the
+# 128 x 128 one; 3000 spins leave a partly filled last spin tile
+@pytest.mark.parametrize("n_wave", [256, 384])
+def test_phase_gemm_matches_fp64(env, n_wave):
+    n, steps, chunk = 3000, 256, 64
     pc = np.float32(2 * np.pi * GAMMA * env["cfg"].dt)
     # 20x the 80 mT/m bound so that the phases reach a sizeable fraction of a radian
     g = PH.random_waveforms(n_wave, steps, seed=11, g_max=1600.0) * np.float32(
@@ -71,6 +74,15 @@ def test_phase_gemm_matches_fp64(env):
     # by ~2^-9
     assert (err <= 5e-5 * scale + 1e-7).all(), float((err / (scale + 1e-30)).max())
     assert np.abs(ref).max() > 0.1           # the test exercises non-trivial phases
+    # the Kahan compensations are values of bf16 (low 16 bits zero) within an
+    # ulp of their own phase (the last fold's rounding error); a pair word
+    # decoded into the wrong spin would break the bound wherever neighbouring
+    # phases differ in magnitude
+    cb = comp.view(np.uint32)
+    assert not (cb & 0xFFFF).any()
+    ulp = np.spacing(np.abs(phi)).astype(np.float64)
+    assert (np.abs(comp.astype(np.float64)) <= ulp * 1.01).all()
+    assert np.count_nonzero(comp) > comp.size // 10   # the compensation is in use
     # signals: FP64 readout of the same phases (x0 term included)
     st = sim.get_state()
     _, x0 = sim.get_moment()
