@@ -405,7 +405,7 @@ def substrate_prep(env):
             "identical": same}
 
 
-def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=1024):
+def waveform_bench(n_spins, n_wave, sample, chunk=64, buffer=2048):
     """Waveform-phase path: n_spins spins on the small substrate, n_wave random
     piecewise-constant waveforms (|G| <= 80 mT/m, a new vector every 10 us) over
     12,000 steps, Kahan-folded every `chunk` steps (GEMM K = 3 * chunk), the
@@ -638,7 +638,7 @@ def main():
                     help="spins of the reference-semantics run (0 skips it)")
     ap.add_argument("--chunk-steps", type=int, default=64,
                     help="Kahan interval of the waveform phases in steps (multiple of 32)")
-    ap.add_argument("--buffer-steps", type=int, default=1024,
+    ap.add_argument("--buffer-steps", type=int, default=2048,
                     help="steps walked per phase-GEMM launch (multiple of --chunk-steps)")
     ap.add_argument("--footprint-axons", default="", metavar="N,N,...",
                     help="only the substrate-footprint measurement at these axon counts")

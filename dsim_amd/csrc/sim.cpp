@@ -36,6 +36,8 @@ enum Field {
 const size_t kMaxEventPairs = 64;
 
 const uint32_t kDefaultStepsPerLaunch = 1024;
+// Longest launch of a walk that records trajectories (sim_walk_traj).
+const uint32_t kTrajLaunchSteps = 128;
 
 }  // namespace
 
@@ -434,6 +436,7 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
     uint32_t done = 0;
     const uint32_t chunk = s->sorting() ? std::min(s->prm.steps_per_launch, s->prm.sort_interval)
                                         : s->prm.steps_per_launch;
+    float *const traj0 = P.traj;  // row 0 of the trajectory buffer's tiles
     while (done < iters) {
         if (s->sorting() && s->steps_since_sort >= s->prm.sort_interval) {
             if (int rc = spatial_sort(s)) return rc;
@@ -447,6 +450,8 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
         P.step0 = s->step + done;
         P.inject = n3u ? s->inject.p + (size_t)done * n : nullptr;
         P.trace = trace ? s->trace.p + (size_t)done * n * 5 : nullptr;
+        // a launch writes its steps' rows 3 * done .. of every tile
+        P.traj = traj0 ? traj0 + (size_t)3 * done * dsim::kTrajTile : nullptr;
         P.stats = instr == dsim::kInstrStats ? s->stats.p : nullptr;
         // each step launch is bracketed by its own events: the timing covers
         // the step kernel only, not the reorders between launches
@@ -784,10 +789,14 @@ int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj) {
     P.weight = 0.0f;
     P.traj = traj;
     P.traj_stride = 3 * iters * dsim::kTrajTile;
-    // one launch for the whole chunk: the kernel writes rows 3*it..3*it+2
+    // Launches of at most kTrajLaunchSteps steps, each writing its steps'
+    // rows: the resident waves of one launch then stand within that many
+    // steps of each other, so their trajectory stores stay in a narrow band
+    // of every tile (one launch per 1024-step buffer walked 15 % slower, and
+    // the gap grows with the buffer).
     const uint32_t spl = s->prm.steps_per_launch;
-    s->prm.steps_per_launch = iters;
-    int rc = run_steps(s, P, iters, rng_kind(s, nullptr), true, nullptr, nullptr);
+    s->prm.steps_per_launch = std::min(spl, kTrajLaunchSteps);
+    const int rc = run_steps(s, P, iters, rng_kind(s, nullptr), true, nullptr, nullptr);
     s->prm.steps_per_launch = spl;
     return rc;
 }
