@@ -2461,7 +2461,7 @@ hipError_t launch_sort_perm(const StepParams &p, uint32_t *keys_a, uint32_t *key
     if (p.n == 0) return hipSuccess;
     const uint32_t tiles = (p.n + kSortTile - 1) / kSortTile;
     size_t need = 0;
-    sort_temp_bytes(p.n, &need);
+    if (const hipError_t e = sort_temp_bytes(p.n, &need); e != hipSuccess) return e;
     if (temp_bytes < need) return hipErrorInvalidValue;
     const uint32_t m = tiles * kRadix, g = (m + kScanBlock - 1) / kScanBlock;
     uint32_t *hist = static_cast<uint32_t *>(temp);
