@@ -207,15 +207,19 @@ def batched_bench(n_substrates, plain_sample, round_steps=256):
 
 
 def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_rate=50.0,
-                 overload_window=8.0, overload_drain=20.0, overload_active_spins=4 << 20):
+                 overload_window=8.0, overload_drain=20.0, overload_active_spins=4 << 20,
+                 small_first_blocks=512):
     """The served configuration on resident crossing substrates.  A stream
     scaled to one GPU (requests of 1e4-2e5 spins, protocols as configured):
     capacity with every request queued at once, the same requests one at a
     time through the plain handle, and Poisson streams at a fraction of the
     capacity for the latency percentiles, each request's latency also against
-    its own solo (plain) time.  Then the configured stream (Poisson 50 req/s,
-    1e4-2e6 spins) over a fixed arrival window: the requests done within the
-    window plus a drain period, their latencies, and the sustained rate."""
+    its own solo (plain) time, under the default round policy and under the
+    small-first one (requests of at most small_first_blocks one-wave blocks
+    walk in rounds of their own), whose capacity is measured too.  Then the
+    configured stream (Poisson 50 req/s, 1e4-2e6 spins) over a fixed arrival
+    window: the requests done within the window plus a drain period, their
+    latencies, and the sustained rate."""
     from dsim_amd import batched as B
 
     reqs = B.stream_requests(cs, envs, n_requests, spins=(1e4, 2e5))
@@ -230,20 +234,34 @@ def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_r
         f"({len(reqs) / solo.sum() * 3600:.0f} simulations/hour); served signals "
         f"{'bit-identical' if same else 'DIFFER'}")
     by_seed = {q["seed"]: t for q, t in zip(reqs, solo)}
+    out_sf, span_sf = B.run_server(cs, envs, reqs, round_steps=round_steps,
+                                   small_first_blocks=small_first_blocks)
+    cap_sf = len(out_sf) / span_sf
+    same_sf = all(np.array_equal(o["signals"], sig) for o, (_, sig) in zip(out_sf, plain))
+    log(f"served, small-first rounds (<= {small_first_blocks} blocks): {cap_sf * 3600:.0f} "
+        f"simulations/hour; signals {'bit-identical' if same_sf else 'DIFFER'}")
     streams = []
     for load in loads:
         rate = load * cap
-        live, _ = B.run_server(cs, envs, B.stream_requests(cs, envs, n_requests, rate=rate,
-                                                           spins=(1e4, 2e5)),
-                               realtime=True, round_steps=round_steps)
-        lat = np.array([o["latency"] for o in live])
-        vs = np.array([o["latency"] / by_seed[o["req"]["seed"]] for o in live])
-        streams.append({"load": load, "rate_per_s": rate, "requests": len(live),
-                        "p50_latency_s": _pct(lat, 50), "p99_latency_s": _pct(lat, 99),
-                        "p50_latency_vs_solo": _pct(vs, 50), "p99_latency_vs_solo": _pct(vs, 99)})
-        log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity), {len(live)} requests: latency "
-            f"p50 {_pct(lat, 50):.3f} s, p99 {_pct(lat, 99):.3f} s; vs solo p50 "
-            f"{_pct(vs, 50):.2f}x, p99 {_pct(vs, 99):.2f}x")
+        stream = B.stream_requests(cs, envs, n_requests, rate=rate, spins=(1e4, 2e5))
+        for policy in (0, small_first_blocks):
+            live, _ = B.run_server(cs, envs, stream, realtime=True, round_steps=round_steps,
+                                   small_first_blocks=policy)
+            lat = np.array([o["latency"] for o in live])
+            vs = np.array([o["latency"] / by_seed[o["req"]["seed"]] for o in live])
+            small = np.array([o["req"]["n"] <= 64 * small_first_blocks for o in live])
+            streams.append({"load": load, "rate_per_s": rate, "requests": len(live),
+                            "small_first_blocks": policy,
+                            "p50_latency_s": _pct(lat, 50), "p99_latency_s": _pct(lat, 99),
+                            "p50_latency_vs_solo": _pct(vs, 50),
+                            "p99_latency_vs_solo": _pct(vs, 99),
+                            "p99_vs_solo_small_requests": _pct(vs[small], 99),
+                            "p99_vs_solo_large_requests": _pct(vs[~small], 99)})
+            log(f"stream at {rate:.1f} req/s ({load:.0%} of capacity), small-first {policy}, "
+                f"{len(live)} requests: latency p50 {_pct(lat, 50):.3f} s, p99 {_pct(lat, 99):.3f} s; "
+                f"vs solo p50 {_pct(vs, 50):.2f}x, p99 {_pct(vs, 99):.2f}x (requests <= "
+                f"{64 * small_first_blocks} spins {_pct(vs[small], 99):.2f}x, larger "
+                f"{_pct(vs[~small], 99):.2f}x)")
     # the latency floor under load: the stream's smallest request submitted
     # together with its k largest, against its solo time
     small = min(reqs, key=lambda q: q["n"])
@@ -294,6 +312,8 @@ def served_bench(cs, envs, n_requests, round_steps=256, loads=(0.4,), overload_r
                          f"{overload_rate:.0f} req/s"),
             "requests": len(reqs), "substrates": len(envs), "round_steps": round_steps,
             "sims_per_hour": cap * 3600, "plain_sims_per_hour": len(reqs) / solo.sum() * 3600,
+            "small_first_sims_per_hour": cap_sf * 3600,
+            "small_first_bit_identical": bool(same_sf),
             "speedup_vs_plain": cap / (len(reqs) / solo.sum()),
             "served_spin_steps_per_s": B.work(reqs) / span,
             # the served rate is the device's batched rate on these substrates

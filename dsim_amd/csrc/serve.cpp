@@ -12,6 +12,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -72,6 +73,7 @@ struct Batch {
 struct dsim_server {
     uint64_t budget = 0, used = 0;
     uint32_t round_steps = 256, max_spins = 0, active_spins = 0, grid_blocks = 0;
+    uint32_t small_first = 0;  // dsim_server_set_policy
     std::vector<dsim_substrate *> subs;
     std::deque<Req *> queue;
     std::vector<Req *> active;
@@ -201,12 +203,20 @@ int run_round(dsim_server *S, uint32_t *n_active, uint32_t *n_queued) {
     // (fewer rounds, lower latency) and go first in the table, so their
     // blocks are dispatched first.
     bool launched = false;
+    // small-first policy: a round with a small prepared request runs only the
+    // small ones
+    uint32_t cap = UINT32_MAX;
+    if (S->small_first)
+        for (Req *r : S->active)
+            if (r->ready && !r->read_out && r->lobe_left && walk_blocks(r->r.n_spins) <= S->small_first)
+                cap = S->small_first;
     for (int kind = 0; kind < 4; ++kind) {
         std::vector<Req *> jobs;
         uint32_t total = 0;
         for (Req *r : S->active) {
             if (!r->ready || r->read_out || r->r.boundary != (uint32_t)(kind >> 1) ||
-                r->lobe_left == 0 || (r->sub->cell_mt.p == nullptr) != ((kind & 1) != 0))
+                r->lobe_left == 0 || (r->sub->cell_mt.p == nullptr) != ((kind & 1) != 0) ||
+                walk_blocks(r->r.n_spins) > cap)
                 continue;
             jobs.push_back(r);
             total += walk_blocks(r->r.n_spins);
@@ -420,6 +430,13 @@ extern "C" int dsim_server_submit(dsim_server *S, const dsim_request *q, uint64_
     S->queue.push_back(r.get());
     S->all.emplace(r->ticket, std::move(r));
     S->cv_work.notify_all();
+    return DSIM_OK;
+}
+
+extern "C" int dsim_server_set_policy(dsim_server *S, uint32_t small_first_blocks) {
+    if (!S) return fail(DSIM_ERR_ARG, "NULL server");
+    std::lock_guard<std::mutex> rl(S->round_mu);
+    S->small_first = small_first_blocks;
     return DSIM_OK;
 }
 

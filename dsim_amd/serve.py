@@ -43,6 +43,7 @@ def _lib():
             "dsim_server_add_substrate": [vp, vp, C.POINTER(C.c_uint32)],
             "dsim_server_submit": [vp, C.POINTER(Request), C.POINTER(C.c_uint64)],
             "dsim_server_round": [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+            "dsim_server_set_policy": [vp, C.c_uint32],
             "dsim_server_start": [vp],
             "dsim_server_stop": [vp],
             "dsim_server_wait": [vp, C.c_uint64, C.c_double],
@@ -70,10 +71,14 @@ def pgse_lobes(L, B, R=0):
 
 
 class Server:
-    def __init__(self, hbm_budget, round_steps=256, max_active_spins=0):
+    def __init__(self, hbm_budget, round_steps=256, max_active_spins=0, small_first_blocks=0):
+        """small_first_blocks: see dsim_server_set_policy (0: every prepared
+        request in every round)."""
         self.h = C.c_void_p()
         _check(_lib().dsim_server_create(int(hbm_budget), round_steps, max_active_spins,
                                          C.byref(self.h)))
+        if small_first_blocks:
+            _check(_lib().dsim_server_set_policy(self.h, int(small_first_blocks)))
         self._subs = []
         self._lines = {}
 

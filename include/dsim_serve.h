@@ -63,6 +63,15 @@ typedef struct {
  * max_active_spins: spins in flight at most (0 = 64M). */
 int dsim_server_create(uint64_t hbm_budget, uint32_t round_steps, uint32_t max_active_spins,
                        dsim_server **out);
+/* Round policy (no counterpart in the reference, which runs one simulation per
+ * process, diffusion.cpp:1068-1184).  small_first_blocks > 0: a round in which
+ * some prepared request needs at most that many one-wave workgroups (64 spins
+ * each) runs only such requests, and the others wait for a round without
+ * one: small requests walk with the device to themselves, near their solo
+ * latency, at the cost of the large requests' latency and of throughput
+ * while they run.  0 (the default): every prepared request takes part in
+ * every round.  Results do not depend on it. */
+int dsim_server_set_policy(dsim_server *srv, uint32_t small_first_blocks);
 /* Stops the background loop if running, waits for the device and frees every
  * request.  Substrates stay owned by the caller. */
 int dsim_server_destroy(dsim_server *srv);

@@ -111,6 +111,35 @@ def test_mix_equals_alone(envs):
     srv.close()
 
 
+def test_small_first_rounds(envs):
+    # a large request of few steps and a small one of many: every round runs
+    # both by default, so the large one ends first; with the small-first
+    # policy (requests of at most 5 one-wave blocks, 320 spins, run alone) the
+    # small one walks in rounds of its own and ends first.  Results equal the
+    # solo runs either way.
+    base = _requests(envs)
+    large = dict(base[0], lobes=SV.pgse_lobes(20, 60, 0))        # 3000 spins, 80 steps
+    small = dict(base[3], lobes=SV.pgse_lobes(200, 600, 100))    # 257 spins, 900 steps
+    reqs = [large, small]
+    alone = _solo(envs, reqs)
+    done = {}
+    for policy in (0, 5):
+        srv = SV.Server(8 << 30, round_steps=100, max_active_spins=1 << 20,
+                        small_first_blocks=policy)
+        ids = [srv.add_substrate(e["sub"]) for e in envs]
+        tickets = [_submit(srv, ids, q) for q in reqs]
+        srv.run_until_idle()
+        ends = []
+        for t, want in zip(tickets, alone):
+            got, st = srv.result(t)
+            np.testing.assert_array_equal(got, want)
+            ends.append(st["done_s"])
+        done[policy] = ends
+        srv.close()
+    assert done[0][0] < done[0][1]      # by default the short large request ends first
+    assert done[5][1] < done[5][0]      # small first: the small request ends first
+
+
 def test_budget_serialises_admission(envs):
     reqs = _requests(envs)
     alone = _solo(envs, reqs)
