@@ -185,14 +185,15 @@ hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint
                              uint32_t k_fold, const uint16_t *g_hi, const uint16_t *g_lo,
                              uint32_t k_total, uint32_t k0, uint32_t n_wave, float *phase,
                              uint32_t *comp, hipStream_t s);
-// Signal of every waveform: per spin block and waveform the FP64 sums of
-// m cos(phase), m sin(phase) and counts for the total and compartments 0-3,
-// phase[s][w] + gsum[w] . x0[s] (in FP64); then a fixed-order final pass
-// into out[n_wave][15].
+// Signal of every waveform: per block of state slots and waveform the FP64
+// sums of m cos(phase), m sin(phase) and counts for the total and
+// compartments 0-3, phase = phase[id[s]][w] + gsum[w] . x0[s] (in FP64; mag,
+// label, x0 and id in slot order, phase by spin id); then a fixed-order
+// final pass into out[n_wave][15].
 hipError_t launch_phase_signal(uint32_t n, const float *phase, uint32_t n_wave,
                                const double *gsum, const float *mag, const uint32_t *label,
-                               const float *x0, double *partial, uint32_t n_blocks,
-                               double *out, hipStream_t s);
+                               const float *x0, const uint32_t *id, double *partial,
+                               uint32_t n_blocks, double *out, hipStream_t s);
 
 // Spatial reordering of spins: Morton key of each slot's cell, radix sort of
 // (key, slot), then a gather of every per-spin field into its new slot.

@@ -1675,6 +1675,7 @@ __global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *
                                                        const float *__restrict__ mag,
                                                        const uint32_t *__restrict__ label,
                                                        const float *__restrict__ x0,
+                                                       const uint32_t *__restrict__ id,
                                                        uint32_t per_block, double *partial) {
     const uint32_t w = blockIdx.x * 256u + threadIdx.x;
     const uint32_t sb = blockIdx.y;
@@ -1684,7 +1685,7 @@ __global__ __launch_bounds__(256) void k_phase_partial(uint32_t n, const float *
     for (int k = 0; k < 15; ++k) v[k] = 0.0;
     const uint32_t s1 = min(n, (sb + 1) * per_block);
     for (uint32_t s = sb * per_block; s < s1; ++s) {
-        const double ph = (double)phase[(size_t)s * n_wave + w] + gx * (double)x0[s] +
+        const double ph = (double)phase[(size_t)id[s] * n_wave + w] + gx * (double)x0[s] +
                           gy * (double)x0[n + s] + gz * (double)x0[2 * n + s];
         double sn, cs;
         sincos(ph, &sn, &cs);
@@ -1708,12 +1709,12 @@ __global__ __launch_bounds__(256) void k_phase_final(const double *partial, uint
 
 hipError_t launch_phase_signal(uint32_t n, const float *phase, uint32_t n_wave,
                                const double *gsum, const float *mag, const uint32_t *label,
-                               const float *x0, double *partial, uint32_t n_blocks,
-                               double *out, hipStream_t s) {
+                               const float *x0, const uint32_t *id, double *partial,
+                               uint32_t n_blocks, double *out, hipStream_t s) {
     if (n_wave == 0 || n_blocks == 0) return hipSuccess;
     const uint32_t per = (n + n_blocks - 1) / n_blocks;
     hipLaunchKernelGGL(k_phase_partial, dim3((n_wave + 255) / 256, n_blocks), dim3(256), 0, s, n,
-                       phase, n_wave, gsum, mag, label, x0, per, partial);
+                       phase, n_wave, gsum, mag, label, x0, id, per, partial);
     hipLaunchKernelGGL(k_phase_final, dim3((n_wave * 15 + 255) / 256), dim3(256), 0, s, partial,
                        n_blocks, n_wave, out);
     return hipGetLastError();

@@ -188,12 +188,12 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
 extern "C" int dsim_waveforms_signal(dsim_waveforms *W, double *out) {
     if (!W || !out) return fail(DSIM_ERR_ARG, "NULL argument");
     const float *mag, *x0;
-    const uint32_t *label;
-    dsim_host::sim_readout_fields(W->sim, &mag, &label, &x0);
+    const uint32_t *label, *id;
+    dsim_host::sim_readout_fields(W->sim, &mag, &label, &x0, &id);
     hipStream_t st = dsim_host::sim_stream(W->sim);
     const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(kPhaseBlocks, W->n));
     hipError_t e = dsim::launch_phase_signal(W->n, W->phase.p, W->n_wave, W->gsum.p, mag, label, x0,
-                                             W->partial.p, nb, W->out.p, st);
+                                             id, W->partial.p, nb, W->out.p, st);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "phase readout: %s", hipGetErrorString(e));
     std::vector<double> v((size_t)W->n_wave * 15);
     HIP_TRY(hipMemcpyAsync(v.data(), W->out.p, v.size() * 8, hipMemcpyDeviceToHost, st));

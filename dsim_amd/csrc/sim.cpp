@@ -443,6 +443,7 @@ static int run_steps(dsim_sim *s, dsim::StepParams P, uint32_t iters, int rng, b
             P = [&] {
                 dsim::StepParams Q = step_params(s);
                 Q.gx = P.gx; Q.gy = P.gy; Q.gz = P.gz; Q.pc = P.pc; Q.weight = P.weight;
+                Q.traj_stride = P.traj_stride;  // traj itself is set per launch below
                 return Q;
             }();
         }
@@ -782,8 +783,8 @@ uint64_t dsim_host::sim_footprint(uint32_t n, const dsim_params *p, uint32_t n_l
 
 int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj) {
     if (int rc = check_sim(s)) return rc;
-    if (s->prm.mode != DSIM_MODE_PHYSICAL || s->sorting())
-        return fail(DSIM_ERR_STATE, "waveform runs need the physical mode without reordering");
+    if (s->prm.mode != DSIM_MODE_PHYSICAL)
+        return fail(DSIM_ERR_STATE, "waveform runs need the physical mode");
     if (!s->moment_ready) return fail(DSIM_ERR_STATE, "call dsim_moment_reset first");
     dsim::StepParams P = step_params(s);
     P.weight = 0.0f;
@@ -802,7 +803,8 @@ int dsim_host::sim_walk_traj(dsim_sim *s, uint32_t iters, float *traj) {
 }
 
 void dsim_host::sim_readout_fields(dsim_sim *s, const float **mag, const uint32_t **label,
-                                   const float **x0) {
+                                   const float **x0, const uint32_t **id) {
+    *id = s->f(F_ID);
     *mag = s->ff(F_MAG);
     *label = s->f(F_LABEL);
     *x0 = s->ff(F_X0);

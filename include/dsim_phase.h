@@ -12,10 +12,12 @@
  * across chunks (hi/lo bf16 split products, FP32 accumulation; the compensation
  * held as bf16), and the signals are summed in FP64.
  *
- * The handle must use the physical mode without spin reordering
- * (sort_interval 0), and dsim_moment_reset must have been called: the phase is
- * pc * sum_t G_w(t) . (x0 + u(t)) with x0 the positions at that reset and u the
- * unwrapped displacement since.
+ * The handle must use the physical mode (spin reordering is allowed, and the
+ * phases follow the spin id, but a reordered wave's trajectory stores scatter
+ * over the buffer: the walk runs several times slower), and
+ * dsim_moment_reset must have been called: the phase
+ * is pc * sum_t G_w(t) . (x0 + u(t)) with x0 the positions at that reset and u
+ * the unwrapped displacement since.
  */
 #ifndef DSIM_PHASE_H
 #define DSIM_PHASE_H

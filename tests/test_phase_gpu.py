@@ -26,11 +26,12 @@ def env():
     return e
 
 
-def _sim(env, n, seed):
+def _sim(env, n, seed, sort_interval=0):
     cfg = env["cfg"]
     sim = D.Simulation(env["sub"], n, dt=cfg.dt, std_dev=env["std_dev"],
                        t2=(90.0, 70.0, 70.0, 70.0), permeability=0.02, mode=D.MODE_PHYSICAL,
-                       boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=seed)
+                       boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=seed,
+                       sort_interval=sort_interval)
     sim.init_uniform(1.0)
     sim.label(env["xmax"], env["xmax"], env["xmax"])
     sim.moment_reset()
@@ -142,4 +143,27 @@ def test_trajectory_buffer_changes_nothing(env):
     np.testing.assert_array_equal(p0, p1)
     np.testing.assert_array_equal(c0, c1)
     np.testing.assert_array_equal(s0, s1)
+    assert np.abs(p0).max() > 0.1
+
+
+def test_reordered_spins_give_the_same_phases(env):
+    # with spatial reordering between walk launches the trajectory rows and
+    # phases follow the spin id: the same phases bit for bit, and the signals
+    # (FP64 sums over slots, in another order) to rounding
+    n, n_wave, steps, chunk = 3000, 256, 512, 64
+    pc = np.float32(2 * np.pi * GAMMA * env["cfg"].dt)
+    g = PH.random_waveforms(n_wave, steps, seed=13, g_max=1600.0) * np.float32(
+        1e-9 * env["space_scale"])
+    out = []
+    for sort in (0, 128):
+        sim = _sim(env, n, seed=24, sort_interval=sort)
+        wf = PH.Waveforms(sim, g, pc, chunk_steps=chunk, buffer_steps=256)
+        wf.run(steps)
+        out.append(wf.phase() + (wf.signal(),))
+        wf.close()
+        sim.close()
+    (p0, c0, s0), (p1, c1, s1) = out
+    np.testing.assert_array_equal(p0, p1)
+    np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_allclose(s0, s1, rtol=0, atol=1e-12)
     assert np.abs(p0).max() > 0.1
