@@ -18,8 +18,10 @@ constexpr uint32_t kFiberNone = 65535u;
 //   MT record:        n.x n.y n.z n.v0 | v0.x v0.y v0.z info | e1 - | e2 -
 // Labelling uses 32-byte AABB records: min.xyz info | max.xyz 0.
 // Spins per tile of the trajectory buffer (StepParams::traj): each tile's rows
-// are contiguous, so a phase GEMM spin tile reads one block.
+// are contiguous, so a phase GEMM spin tile reads one block; within a tile
+// the k (3 * step + axis) of a spin come in 16-byte chunks of kTrajChunk.
 constexpr uint32_t kTrajTile = 128;
+constexpr uint32_t kTrajChunk = 4;
 
 struct StepParams {
     // substrate (device pointers)
@@ -61,8 +63,9 @@ struct StepParams {
                       // for the phase of the unwrapped position when moments are off
     uint32_t hit_window;  // steps those counts cover (0: no history)
     float *traj;          // displacement after each step of the launch by spin id, in
-                          // tiles of kTrajTile spins: row r (3 * step + axis) of spin
-                          // s at [s / kTrajTile][r][s % kTrajTile] (moment mode), or null
+                          // tiles of kTrajTile spins: k = 3 * step + axis of spin s at
+                          // [s / kTrajTile][k / kTrajChunk][s % kTrajTile][k % kTrajChunk]
+                          // (moment mode), or null
     uint32_t traj_stride;  // floats per tile: 3 * iters * kTrajTile
     // per call
     float gx, gy, gz, pc, weight;

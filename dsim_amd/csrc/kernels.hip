@@ -961,7 +961,11 @@ __device__ __forceinline__ uint32_t queue_slot(uint32_t q, uint32_t g, uint32_t 
     return s < n ? (uint32_t)s : n;
 }
 
-template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS, bool QUEUE = false>
+// TRAJBUF (trajectory runs): each lane keeps its last four steps'
+// displacements in registers and stores them as three 16-byte chunks every
+// fourth step (launches and buffers are multiples of four steps).
+template <int BOUND, int RNG, int INSTR, bool MOMENT, int RECS, bool QUEUE = false,
+          bool TRAJBUF = false>
 __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, uint32_t iters,
                                           WaveScratch &L) {
     constexpr bool PERIODIC = BOUND != kBoundClamp;
@@ -975,6 +979,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
     float x = 0.f, y = 0.f, z = 0.f, mag = 0.f, ph = 0.f;
     float dx = 0.f, dy = 0.f, dz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     float safe = -1.0f;
+    float tb[TRAJBUF ? 12 : 1];  // TRAJBUF: displacements of the last four steps
     // voxel crossings (periodic phase): the unwrapped position is x + kx * period
     int kx = 0, ky = 0, kz = 0;
     uint32_t nhit = 0, hit_total = 0;
@@ -1157,14 +1162,32 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
                 yv = fmaf(P.weight, dy, -cy); tv = my + yv; cy = (tv - my) - yv; my = tv;
                 yv = fmaf(P.weight, dz, -cz); tv = mz + yv; cz = (tv - mz) - yv; mz = tv;
             }
-            if (MOMENT && P.traj) {
-                // unwrapped displacement after this step, rows 3*it + axis
+            if (TRAJBUF) {
+                // unwrapped displacement after this step, k = 3 * it + axis,
+                // shifted into the four-step window; stored when it is full
+#pragma unroll
+                for (int q = 0; q < 9; ++q) tb[q] = tb[q + 3];
+                tb[9] = dx; tb[10] = dy; tb[11] = dz;
+                if ((it & 3u) == 3u) {
+                    float4 *tr = reinterpret_cast<float4 *>(
+                        P.traj + (size_t)(sid / kTrajTile) * P.traj_stride +
+                        (size_t)(3 * (it - 3) / kTrajChunk) * (kTrajTile * kTrajChunk)) +
+                        sid % kTrajTile;
+                    tr[0] = make_float4(tb[0], tb[1], tb[2], tb[3]);
+                    tr[kTrajTile] = make_float4(tb[4], tb[5], tb[6], tb[7]);
+                    tr[2 * kTrajTile] = make_float4(tb[8], tb[9], tb[10], tb[11]);
+                }
+            } else if (MOMENT && P.traj) {
+                // unwrapped displacement after this step, k = 3 * it + axis
                 // of the spin's tile
                 float *tr = P.traj + (size_t)(sid / kTrajTile) * P.traj_stride +
-                            (3 * it) * kTrajTile + sid % kTrajTile;
-                tr[0] = dx;
-                tr[kTrajTile] = dy;
-                tr[2 * kTrajTile] = dz;
+                            (sid % kTrajTile) * kTrajChunk;
+#pragma unroll
+                for (uint32_t a = 0; a < 3; ++a) {
+                    const uint32_t k = 3 * it + a;
+                    tr[(size_t)(k / kTrajChunk) * (kTrajTile * kTrajChunk) + k % kTrajChunk] =
+                        a == 0 ? dx : a == 1 ? dy : dz;
+                }
             }
             if (TRACE) {
                 uint32_t qx, qy, qz;
@@ -1215,6 +1238,16 @@ __global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu
     StepParams P, uint32_t iters) {
     __shared__ WaveScratch scratch[1];
     walk_body<BOUND, RNG, INSTR, MOMENT, RECS>(P, xcd_block(kWalkXcdRun), iters, scratch[0]);
+}
+
+// The walk of trajectory runs (moment mode, Philox): k_walk with the
+// trajectory stores gathered four steps at a time (walk_body's TRAJBUF).
+template <int BOUND, int RECS>
+__global__ __launch_bounds__(kWalkBlockSpins) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_traj(
+    StepParams P, uint32_t iters) {
+    __shared__ WaveScratch scratch[1];
+    walk_body<BOUND, kRngPhilox, kInstrNone, true, RECS, false, true>(P, xcd_block(kWalkXcdRun),
+                                                                      iters, scratch[0]);
 }
 
 // Persistent walk: one one-wave workgroup per resident slot (block b on XCD
@@ -1294,6 +1327,20 @@ static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
                                    dim3(kWalkBlockSpins), 0, s, p, iters);
             else
                 hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsIndirect>), dim3(slots),
+                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+            return;
+        }
+    }
+    if constexpr (MODE == 1 && MOMENT && RNG == kRngPhilox && INSTR == kInstrNone) {
+        // four-step windows need whole windows per launch
+        if (p.traj && iters % 4 == 0) {
+            if (p.cell_mt)
+                hipLaunchKernelGGL((k_walk_traj<BOUND, kRecsCellMajor>),
+                                   dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
+                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+            else
+                hipLaunchKernelGGL((k_walk_traj<BOUND, kRecsIndirect>),
+                                   dim3((p.n + kWalkBlockSpins - 1) / kWalkBlockSpins),
                                    dim3(kWalkBlockSpins), 0, s, p, iters);
             return;
         }
@@ -1427,12 +1474,16 @@ __device__ __forceinline__ void phase_gload(PhaseStage<TJ> &S, const float *__re
                                             const uint16_t *__restrict__ g_hi,
                                             const uint16_t *__restrict__ g_lo, uint32_t n0,
                                             uint32_t k_total, uint32_t k0, uint32_t t) {
+    // k = ks + akg .. + 7 of spin am: two 16-byte chunks of kTrajChunk k
     const __amdgpu_buffer_rsrc_t ra =
         raw_rsrc(traj + ((size_t)tm * k_len + ks + akg) * kTrajTile);
 #pragma unroll
-    for (uint32_t q = 0; q < 8; ++q)
-        S.tv[q] = __uint_as_float(
-            __builtin_amdgcn_raw_buffer_load_b32(ra, am * 4u, q * kTrajTile * 4u, 0));
+    for (uint32_t c = 0; c < 2; ++c) {
+        const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(
+            ra, am * 16u, c * kTrajTile * kTrajChunk * 4u, 0);
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) S.tv[4 * c + q] = __uint_as_float(v[q]);
+    }
     const size_t gb = (size_t)n0 * k_total + k0 + ks;
     const __amdgpu_buffer_rsrc_t rh = raw_rsrc(g_hi + gb), rl = raw_rsrc(g_lo + gb);
 #pragma unroll

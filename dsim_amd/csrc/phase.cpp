@@ -156,19 +156,18 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
         }
         HIP_TRY(hipEventRecord(ev[2], st));
         if (traj_out) {
-            // rows [3 * len][n] out of the spin tiles
+            // rows [3 * len][n] out of the spin tiles' 16-byte k chunks
             const size_t tile = (size_t)3 * len * dsim::kTrajTile;  // floats per tile
-            const uint32_t full = W->n / dsim::kTrajTile, rest = W->n % dsim::kTrajTile;
+            std::vector<float> h(tile * (W->n_pad / dsim::kTrajTile));
+            HIP_TRY(hipMemcpyAsync(h.data(), W->traj.p, h.size() * 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
             for (uint32_t r = 0; r < 3 * len; ++r) {
                 float *dst = traj_out + ((size_t)3 * c + r) * W->n;
-                const float *src = W->traj.p + (size_t)r * dsim::kTrajTile;
-                if (full)
-                    HIP_TRY(hipMemcpy2DAsync(dst, dsim::kTrajTile * 4, src, tile * 4,
-                                             dsim::kTrajTile * 4, full, hipMemcpyDeviceToHost, st));
-                if (rest)
-                    HIP_TRY(hipMemcpyAsync(dst + (size_t)full * dsim::kTrajTile,
-                                           src + full * tile, (size_t)rest * 4,
-                                           hipMemcpyDeviceToHost, st));
+                const size_t kc = (size_t)(r / dsim::kTrajChunk) * dsim::kTrajTile * dsim::kTrajChunk +
+                                  r % dsim::kTrajChunk;
+                for (uint32_t s = 0; s < W->n; ++s)
+                    dst[s] = h[(s / dsim::kTrajTile) * tile + kc +
+                               (size_t)(s % dsim::kTrajTile) * dsim::kTrajChunk];
             }
         }
         HIP_TRY(hipEventSynchronize(ev[2]));
