@@ -1531,7 +1531,7 @@ __device__ __forceinline__ void phase_lstore(const PhaseStage<TJ> &S, uint16_t *
 // slice.
 template <int TJ>
 __global__ __launch_bounds__(kPhaseThreads) void k_phase_gemm(
-    const float *__restrict__ traj, uint32_t n, uint32_t n_pad, uint32_t k_len, uint32_t k_fold,
+    const float *__restrict__ traj, uint32_t n_pad, uint32_t k_len, uint32_t k_fold,
     const uint16_t *__restrict__ g_hi, const uint16_t *__restrict__ g_lo, uint32_t k_total,
     uint32_t k0, uint32_t n_wave, float *__restrict__ phase, uint32_t *__restrict__ comp) {
     constexpr uint32_t kTileN = 128u * TJ;
@@ -1710,10 +1710,10 @@ hipError_t launch_phase_gemm(const float *traj, uint32_t n, uint32_t n_pad, uint
     const uint32_t m_groups = (n_pad / kPhaseTileM + 7u) / 8u;
     const uint32_t grid = m_groups * 8u * (n_wave / tile_n);
     if (wide)
-        hipLaunchKernelGGL(k_phase_gemm<2>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
+        hipLaunchKernelGGL(k_phase_gemm<2>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n_pad,
                            k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phase, comp);
     else
-        hipLaunchKernelGGL(k_phase_gemm<1>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n, n_pad,
+        hipLaunchKernelGGL(k_phase_gemm<1>, dim3(grid), dim3(kPhaseThreads), 0, s, traj, n_pad,
                            k_len, k_fold, g_hi, g_lo, k_total, k0, n_wave, phase, comp);
     return hipGetLastError();
 }
