@@ -26,11 +26,11 @@ def env():
     return e
 
 
-def _sim(env, n, seed, sort_interval=0):
+def _sim(env, n, seed, sort_interval=0, rng=D.RNG_PHILOX):
     cfg = env["cfg"]
     sim = D.Simulation(env["sub"], n, dt=cfg.dt, std_dev=env["std_dev"],
                        t2=(90.0, 70.0, 70.0, 70.0), permeability=0.02, mode=D.MODE_PHYSICAL,
-                       boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=seed,
+                       boundary=D.BOUNDARY_PERIODIC, rng=rng, seed=seed,
                        sort_interval=sort_interval)
     sim.init_uniform(1.0)
     sim.label(env["xmax"], env["xmax"], env["xmax"])
@@ -96,7 +96,10 @@ def test_phase_gemm_matches_fp64(env, n_wave):
     sim.close()
 
 
-def test_pgse_waveform_matches_moment_path(env):
+# Philox walks record trajectories four steps at a time (k_walk_traj); MWC
+# walks store every step (the walk's general path)
+@pytest.mark.parametrize("rng_kind", [D.RNG_PHILOX, D.RNG_MWC])
+def test_pgse_waveform_matches_moment_path(env, rng_kind):
     n, steps, chunk = 4000, 320, 64
     L, B = 64, 192
     pc = np.float32(2 * np.pi * GAMMA * env["cfg"].dt)
@@ -108,13 +111,13 @@ def test_pgse_waveform_matches_moment_path(env):
     w[:L] = 1.0
     w[B:B + L] = -1.0
     g = w[:, None, None] * G[None, :, :]
-    sim = _sim(env, n, seed=33)
+    sim = _sim(env, n, seed=33, rng=rng_kind)
     wf = PH.Waveforms(sim, g, pc, chunk_steps=chunk)
     wf.run(steps)
     got = wf.signal()
     wf.close()
     sim.close()
-    ref_sim = _sim(env, n, seed=33)
+    ref_sim = _sim(env, n, seed=33, rng=rng_kind)
     for wt, k in [(1.0, L), (0.0, B - L), (-1.0, L), (0.0, steps - B - L)]:
         ref_sim.integrate_moment(wt, k)
     want = ref_sim.signal_moment(G, pc)
