@@ -22,12 +22,13 @@ using dsim_host::upload;
 struct dsim_waveforms {
     dsim_sim *sim = nullptr;
     uint32_t n = 0, n_pad = 0, n_wave = 0, n_steps = 0, chunk = 0, done = 0;
+    uint32_t n_wpad = 0;          // waveforms padded with zero ones to a multiple of kPhaseTileN
     uint32_t buffer = 0;          // steps walked per trajectory buffer (a multiple of chunk)
-    DevBuf<uint16_t> g_hi, g_lo;  // [n_wave][3*n_steps] bf16 bits, pc folded in
-    DevBuf<double> gsum;          // [n_wave][3]: sum_t pc * G_w(t)
+    DevBuf<uint16_t> g_hi, g_lo;  // [n_wpad][3*n_steps] bf16 bits, pc folded in
+    DevBuf<double> gsum;          // [n_wpad][3]: sum_t pc * G_w(t)
     DevBuf<float> traj;           // n_pad / kTrajTile tiles of [3*len][kTrajTile] (len <= buffer)
-    DevBuf<float> phase;          // [n_pad][n_wave] (rows past n: padding)
-    DevBuf<uint32_t> comp;        // [n_pad / 2][n_wave]: Kahan compensation of spins 2m, 2m + 1
+    DevBuf<float> phase;          // [n_pad][n_wpad] (rows past n: padding)
+    DevBuf<uint32_t> comp;        // [n_pad / 2][n_wpad]: Kahan compensation of spins 2m, 2m + 1
                                   // as bf16 in the low and high half
     DevBuf<double> partial, out;
     double walk_ms = 0.0, gemm_ms = 0.0;
@@ -63,8 +64,7 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
                                      dsim_waveforms **out) {
     if (!sim || !grads || !out) return fail(DSIM_ERR_ARG, "NULL argument");
     *out = nullptr;
-    if (n_wave == 0 || n_wave % dsim::kPhaseTileN)
-        return fail(DSIM_ERR_ARG, "n_wave must be a positive multiple of %u", dsim::kPhaseTileN);
+    if (n_wave == 0) return fail(DSIM_ERR_ARG, "n_wave must be positive");
     if (chunk == 0 || chunk % dsim::kPhaseSliceK || n_steps == 0 || n_steps % dsim::kPhaseSliceK)
         return fail(DSIM_ERR_ARG, "chunk_steps and n_steps must be multiples of %u",
                     dsim::kPhaseSliceK);
@@ -73,12 +73,16 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
     W->n = dsim_host::sim_spins(sim);
     W->n_pad = (W->n + dsim::kPhaseTileM - 1) / dsim::kPhaseTileM * dsim::kPhaseTileM;
     W->n_wave = n_wave;
+    // the GEMM's waveform tiles take 128 waveforms (256 where they divide into
+    // those); zero waveforms fill the last tile and are never read out
+    const uint32_t nwp = (n_wave + dsim::kPhaseTileN - 1) / dsim::kPhaseTileN * dsim::kPhaseTileN;
+    W->n_wpad = nwp;
     W->n_steps = n_steps;
     W->chunk = chunk;
     W->buffer = chunk;
     const size_t K = (size_t)3 * n_steps;
-    std::vector<uint16_t> hi(n_wave * K), lo(n_wave * K);
-    std::vector<double> gs((size_t)n_wave * 3, 0.0);
+    std::vector<uint16_t> hi(nwp * K, 0), lo(nwp * K, 0);
+    std::vector<double> gs((size_t)nwp * 3, 0.0);
     for (uint32_t t = 0; t < n_steps; ++t)
         for (uint32_t w = 0; w < n_wave; ++w)
             for (uint32_t a = 0; a < 3; ++a) {
@@ -89,12 +93,12 @@ extern "C" int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_
                 gs[(size_t)w * 3 + a] += (double)v;
             }
     int rc;
-    const size_t nw = (size_t)std::max<uint32_t>(W->n_pad, 1) * n_wave;
+    const size_t nw = (size_t)std::max<uint32_t>(W->n_pad, 1) * nwp;
     if ((rc = upload(W->g_hi, hi)) || (rc = upload(W->g_lo, lo)) || (rc = upload(W->gsum, gs)) ||
         (rc = W->traj.alloc((size_t)3 * chunk * W->n_pad)) || (rc = W->phase.alloc(nw)) ||
-        (rc = W->comp.alloc((size_t)std::max<uint32_t>(W->n_pad / 2, 1) * n_wave)) ||
-        (rc = W->partial.alloc((size_t)kPhaseBlocks * n_wave * 15)) ||
-        (rc = W->out.alloc((size_t)n_wave * 15))) {
+        (rc = W->comp.alloc((size_t)std::max<uint32_t>(W->n_pad / 2, 1) * nwp)) ||
+        (rc = W->partial.alloc((size_t)kPhaseBlocks * nwp * 15)) ||
+        (rc = W->out.alloc((size_t)nwp * 15))) {
         delete W;
         return rc;
     }
@@ -148,7 +152,7 @@ extern "C" int dsim_waveforms_run(dsim_waveforms *W, uint32_t steps, float *traj
         HIP_TRY(hipEventRecord(ev[1], st));
         hipError_t e = dsim::launch_phase_gemm(W->traj.p, W->n, W->n_pad, 3 * len, 3 * W->chunk,
                                                W->g_hi.p, W->g_lo.p, 3 * W->n_steps,
-                                               3 * W->done, W->n_wave, W->phase.p, W->comp.p,
+                                               3 * W->done, W->n_wpad, W->phase.p, W->comp.p,
                                                st);
         if (e != hipSuccess) {
             rc = fail(DSIM_ERR_HIP, "phase GEMM: %s", hipGetErrorString(e));
@@ -191,10 +195,10 @@ extern "C" int dsim_waveforms_signal(dsim_waveforms *W, double *out) {
     dsim_host::sim_readout_fields(W->sim, &mag, &label, &x0, &id);
     hipStream_t st = dsim_host::sim_stream(W->sim);
     const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(kPhaseBlocks, W->n));
-    hipError_t e = dsim::launch_phase_signal(W->n, W->phase.p, W->n_wave, W->gsum.p, mag, label, x0,
+    hipError_t e = dsim::launch_phase_signal(W->n, W->phase.p, W->n_wpad, W->gsum.p, mag, label, x0,
                                              id, W->partial.p, nb, W->out.p, st);
     if (e != hipSuccess) return fail(DSIM_ERR_HIP, "phase readout: %s", hipGetErrorString(e));
-    std::vector<double> v((size_t)W->n_wave * 15);
+    std::vector<double> v((size_t)W->n_wpad * 15);
     HIP_TRY(hipMemcpyAsync(v.data(), W->out.p, v.size() * 8, hipMemcpyDeviceToHost, st));
     if (int rc = dsim_sync(W->sim)) return rc;
     for (uint32_t w = 0; w < W->n_wave; ++w) {
@@ -210,13 +214,16 @@ extern "C" int dsim_waveforms_get_phase(dsim_waveforms *W, float *phase, float *
     if (!W) return fail(DSIM_ERR_ARG, "NULL waveforms");
     if (int rc = dsim_sync(W->sim)) return rc;
     const size_t m = (size_t)W->n * W->n_wave;
-    if (phase && m) HIP_TRY(hipMemcpy(phase, W->phase.p, m * 4, hipMemcpyDeviceToHost));
+    const uint32_t nwp = W->n_wpad;  // row stride of the device arrays
+    if (phase && m)
+        HIP_TRY(hipMemcpy2D(phase, (size_t)W->n_wave * 4, W->phase.p, (size_t)nwp * 4,
+                            (size_t)W->n_wave * 4, W->n, hipMemcpyDeviceToHost));
     if (comp && m) {
-        std::vector<uint32_t> v((size_t)(W->n + 1) / 2 * W->n_wave);
+        std::vector<uint32_t> v((size_t)(W->n + 1) / 2 * nwp);
         HIP_TRY(hipMemcpy(v.data(), W->comp.p, v.size() * 4, hipMemcpyDeviceToHost));
         for (uint32_t s = 0; s < W->n; ++s)
             for (uint32_t w = 0; w < W->n_wave; ++w) {
-                const uint32_t word = v[(size_t)(s / 2) * W->n_wave + w];
+                const uint32_t word = v[(size_t)(s / 2) * nwp + w];
                 const uint32_t bits = (s & 1u) ? (word & 0xffff0000u) : (word << 16);
                 std::memcpy(&comp[(size_t)s * W->n_wave + w], &bits, 4);
             }

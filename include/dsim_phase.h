@@ -32,15 +32,17 @@ extern "C" {
 
 typedef struct dsim_waveforms dsim_waveforms;
 
-/* n_wave: a multiple of 128; n_steps and chunk_steps: multiples of 32.
+/* n_wave: any positive count (the device arrays are padded with zero
+ * waveforms to a multiple of 128); n_steps and chunk_steps: multiples of 32.
  * chunk_steps is the Kahan interval: the products of each chunk (GEMM K =
  * 3 * chunk_steps) are summed in FP32 and then added to the phase with
  * compensation; the last chunk of a run may be shorter.  grads: host array
  * [n_steps][n_wave][3] of gradScaled values (include/dsim.h units),
  * phase_constant = 2*pi*gamma*dt.  Device memory: 6 bytes per (spin,
- * waveform) for the phase (FP32) and its compensation (bf16), spins padded to
- * a multiple of 128, plus 4 * n_wave * 3 * n_steps bytes of bf16 gradients and
- * one trajectory buffer (12 bytes per spin and buffered step). */
+ * waveform) for the phase (FP32) and its compensation (bf16), spins and
+ * waveforms padded to multiples of 128, plus 4 * n_wave * 3 * n_steps bytes of
+ * bf16 gradients and one trajectory buffer (12 bytes per spin and buffered
+ * step). */
 int dsim_waveforms_create(dsim_sim *sim, uint32_t n_wave, uint32_t n_steps, const float *grads,
                           float phase_constant, uint32_t chunk_steps, dsim_waveforms **out);
 int dsim_waveforms_destroy(dsim_waveforms *wf);

@@ -50,8 +50,9 @@ def _fp64_signal(phase, mag, comp):
 
 
 # 256 waveforms take the 128 x 256 workgroup tile, 384 (three tiles of 128) the
-# 128 x 128 one; 3000 spins leave a partly filled last spin tile
-@pytest.mark.parametrize("n_wave", [256, 384])
+# 128 x 128 one, 200 are padded with zero waveforms to 256; 3000 spins leave a
+# partly filled last spin tile
+@pytest.mark.parametrize("n_wave", [256, 384, 200])
 def test_phase_gemm_matches_fp64(env, n_wave):
     n, steps, chunk = 3000, 256, 64
     pc = np.float32(2 * np.pi * GAMMA * env["cfg"].dt)
