@@ -956,6 +956,8 @@ enum { kBoundClamp = 0, kBoundCube = 1, kBoundBox = 2 };
 // that every XCD walks a like mix of the sorted order and a compact stretch
 // of it at a time.  Returns n (no slot) past the end.
 constexpr uint32_t kWalkQueueRun = 8192;
+// Shortest launch (steps) that takes the persistent walk (see launch_one).
+constexpr uint32_t kQueueMinSteps = 768;
 __device__ __forceinline__ uint32_t queue_slot(uint32_t q, uint32_t g, uint32_t n) {
     const uint64_t s = ((uint64_t)(g / kWalkQueueRun) * kXcds + q) * kWalkQueueRun + g % kWalkQueueRun;
     return s < n ? (uint32_t)s : n;
@@ -1316,11 +1318,16 @@ static uint32_t resident_walk_blocks() {
 template <int MODE, int BOUND, int RNG, int INSTR, bool MOMENT>
 static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
     if constexpr (MODE == 1 && RNG == kRngPhilox && INSTR != kInstrTrace) {
-        // the persistent walk once every resident slot has two spins or more;
-        // not while a trajectory is recorded (its lanes, at different steps of
-        // different spins, would scatter every trajectory store over 64 rows)
+        // the persistent walk once every resident slot has two spins or more
+        // and a launch walks at least kQueueMinSteps steps; not while a
+        // trajectory is recorded (its lanes, at different steps of different
+        // spins, would scatter every trajectory store over 64 rows).  Lanes
+        // take a queue slot per spin walked, so short launches turn the eight
+        // queue counters into a stream of atomics: 4e6 spins in 128-step
+        // launches walked 2.3x slower than fixed blocks, at 512 steps 3 %
+        // slower, at 768 3 % faster.
         const uint32_t slots = resident_walk_blocks();
-        if (p.queue && !p.traj && p.n >= 2 * slots * kWalkBlockSpins) {
+        if (p.queue && !p.traj && p.n >= 2 * slots * kWalkBlockSpins && iters >= kQueueMinSteps) {
             if (hipMemsetAsync(p.queue, 0, kXcds * sizeof(uint32_t), s) != hipSuccess) return;
             if (p.cell_mt)
                 hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsCellMajor>), dim3(slots),

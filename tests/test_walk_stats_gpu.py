@@ -115,11 +115,13 @@ def test_counting_refused_for_mwc(ref_inputs):
 
 
 def test_persistent_walk_equals_fixed_blocks(ref_inputs):
-    # 6e5 spins: at least two per resident one-wave workgroup, so the
-    # persistent walk (lanes taking their next spin from a per-XCD queue)
-    # runs; its results and counts equal the fixed-block walk's
+    # 6e5 spins: at least two per resident one-wave workgroup, and launches of
+    # 1024 steps (the persistent walk takes launches of 768 or more), so the
+    # persistent walk (lanes taking their next spin from a per-XCD queue) runs,
+    # twice, with a reorder between; its results and counts equal the
+    # fixed-block walk's
     _say("case persistent walk")
-    n, iters = 600_000, 200
+    n, iters = 600_000, 2048
     m = O.parse_mesh(os.path.join(ref_inputs, "sim.fibers_1_r004_f09"), 6.0)
     dims, lo, cell = (37, 29, 13), (-1.2, -1.2, -1.2), (2.4 / 37, 2.4 / 29, 2.4 / 13)
     sub = D.Substrate(m["vertices"], m["trgls"], m["tri_info"], grid="aabb", dims=dims, lo=lo,
@@ -129,7 +131,7 @@ def test_persistent_walk_equals_fixed_blocks(ref_inputs):
         sim = D.Simulation(sub, n, dt=DT, std_dev=(0.03, 0.03, 0.01, 0.03),
                            t2=(80.0, 70.0, 7.5, 70.0), permeability=0.2, mode=D.MODE_PHYSICAL,
                            boundary=D.BOUNDARY_PERIODIC, rng=D.RNG_PHILOX, seed=7,
-                           sort_interval=64, fixed_blocks=fixed)
+                           sort_interval=1024, fixed_blocks=fixed)
         sim.init_uniform(0.9)
         sim.label(m["xmax"], m["ymax"], m["zmax"])
         sim.moment_reset()
