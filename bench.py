@@ -692,7 +692,7 @@ def main():
     cell_um, sort_interval = args.cell_um or 0.25, 128 if args.sort < 0 else args.sort
     if not args.no_autotune and not args.cell_um:
         t0 = time.perf_counter()
-        best, allr = AT.tune(cfg, axons, n, sorts=(0, 256, 512, 1024) if args.sort < 0 else (args.sort,),
+        best, allr = AT.tune(cfg, axons, n, sorts=AT.SORTS if args.sort < 0 else (args.sort,),
                              log=log)
         cell_um, sort_interval = best["cell_um"], best["sort_interval"]
         tuned = dict(cell_um=cell_um, sort_interval=sort_interval,

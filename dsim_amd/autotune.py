@@ -52,15 +52,20 @@ def time_steps(sim, steps, warm=64):
     return ms / 1e3
 
 
-def tune(cfg, axons, n, *, cells_um=(0.15, 0.2, 0.25), sorts=(0, 256, 512, 1024),
+SORTS = (0, 256, 512, 1024, 2048, 4096)
+
+
+def tune(cfg, axons, n, *, cells_um=(0.15, 0.2, 0.25), sorts=SORTS,
          steps=1024, warm=320, tie_tol=0.01, log=None):
     """Returns (best dict, all results).  Each result: cell_um, sort_interval,
     spin-steps/s, grid dims.  Among the candidates within tie_tol of the
     fastest (run-to-run timing noise), the one with the longest reorder
     interval, then the largest cell (fewer reorders, smaller grid) is chosen,
     so that repeated runs settle on the same configuration.  Each candidate
-    launches launch_steps(sort) steps at a time, and its warm-up runs past the
-    first reorder keyed by measured hit rates."""
+    launches launch_steps(sort) steps at a time, its warm-up runs past the
+    first reorder keyed by measured hit rates, and it is timed over at least
+    `steps` and two of its reorder intervals (whole launches and their
+    reorders)."""
     results = []
     for h in cells_um:
         env = S.build(cfg, cell_um=h, axons=axons)
@@ -70,8 +75,9 @@ def tune(cfg, axons, n, *, cells_um=(0.15, 0.2, 0.25), sorts=(0, 256, 512, 1024)
             sim.init_uniform(1.0)
             sim.label(env["xmax"], env["xmax"], env["xmax"])
             # warm up past the first reorder keyed by measured hit rates
-            sec = time_steps(sim, steps, warm=max(warm, srt + 64))
-            rate = n * steps / sec
+            timed = max(steps, 2 * srt)
+            sec = time_steps(sim, timed, warm=max(warm, srt + 64))
+            rate = n * timed / sec
             r = dict(cell_um=h, sort_interval=srt, rate=rate, grid=list(env["grid"]["dims"]),
                      setup_s=time.perf_counter() - t0 - sec)
             results.append(r)
