@@ -2466,7 +2466,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     uint32_t n_tiles, const uint32_t *__restrict__ offsets) {
     __shared__ uint32_t run[kRadix];                        // next free place per digit
     __shared__ uint32_t wave_cnt[kSortThreads / 64][kRadix];  // per-wave digit counts
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, tile = blockIdx.x;
+    const uint32_t t = threadIdx.x, w = t >> 6, tile = blockIdx.x;
     run[t] = offsets[t * n_tiles + tile];
     for (uint32_t q = 0; q < kSortThreads / 64; ++q) wave_cnt[q][t] = 0;
     __syncthreads();
