@@ -76,6 +76,7 @@ struct StepParams {
     uint32_t *flags;       // status word: bit 0 collision loop guard hit, bit 2 entry out of range
     unsigned long long *stats;  // [kWalkStats] work counters (counting build of the walk)
     uint32_t *queue;       // [8] slot counters of the persistent walk, or null (fixed blocks)
+    uint32_t queue_run;    // slots per queue run (persistent walk; set at launch)
 };
 
 struct LabelParams {

@@ -952,16 +952,30 @@ __device__ __forceinline__ uint32_t pack_wrap(int kx, int ky, int kz) {
 enum { kBoundClamp = 0, kBoundCube = 1, kBoundBox = 2 };
 
 // Spin slots of the persistent walk: queue q (one per XCD) hands out the
-// slots of runs q, q + 8, q + 16, ... of kWalkQueueRun consecutive slots, so
-// that every XCD walks a like mix of the sorted order and a compact stretch
-// of it at a time.  Returns n (no slot) past the end.
-constexpr uint32_t kWalkQueueRun = 8192;
-// Shortest launch (steps) that takes the persistent walk (see launch_one).
-constexpr uint32_t kQueueMinSteps = 768;
-__device__ __forceinline__ uint32_t queue_slot(uint32_t q, uint32_t g, uint32_t n) {
-    const uint64_t s = ((uint64_t)(g / kWalkQueueRun) * kXcds + q) * kWalkQueueRun + g % kWalkQueueRun;
+// slots of runs q, q + 8, q + 16, ... of `run` consecutive slots, so that
+// every XCD walks a like mix of the sorted order and a compact stretch of it
+// at a time.  Slot s = (8 (g / run) + q) run + g % run grows with the queue
+// count g, so a queue past the end stays there.  Returns n (no slot) past
+// the end.
+__device__ __forceinline__ uint32_t queue_slot(uint32_t q, uint32_t g, uint32_t n, uint32_t run) {
+    const uint64_t s = ((uint64_t)(g / run) * kXcds + q) * run + g % run;
     return s < n ? (uint32_t)s : n;
 }
+// Run length for n spins: as near kWalkQueueRun as gives every XCD the same
+// number of runs (the last one short).  The queues do not share work, so
+// with a fixed length the XCDs holding one run fewer than the others sit
+// idle at the end of every launch: 1e6 spins in runs of 8192 are 122 runs,
+// 16 for two XCDs and 15 for six.  On 1e6 spins at a 2048-step reorder
+// interval: fixed runs of 8192 2.03e10 spin-steps/s, of 4096 2.10e10;
+// balanced runs near 4096 2.10e10, near 2048 2.12e10, near 1024 2.09e10.
+constexpr uint32_t kWalkQueueRun = 2048;
+inline uint32_t queue_run(uint32_t n) {
+    const uint64_t per_xcd =
+        std::max<uint64_t>(1, ((uint64_t)n + kXcds * kWalkQueueRun - 1) / (kXcds * kWalkQueueRun));
+    return (uint32_t)(((uint64_t)n + kXcds * per_xcd - 1) / (kXcds * per_xcd));
+}
+// Shortest launch (steps) that takes the persistent walk (see launch_one).
+constexpr uint32_t kQueueMinSteps = 768;
 
 // TRAJBUF (trajectory runs): each lane keeps its last four steps'
 // displacements in registers and stores them as three 16-byte chunks every
@@ -1026,7 +1040,7 @@ __device__ __forceinline__ void walk_body(const StepParams &P, uint32_t block, u
         const uint32_t lead = (uint32_t)__builtin_ctzll(m);
         if (lane == lead) base = atomicAdd(P.queue + q, popc(m));
         base = __builtin_amdgcn_readlane(base, lead);
-        return want ? queue_slot(q, base + lane_rank(m), P.n) : P.n;
+        return want ? queue_slot(q, base + lane_rank(m), P.n, P.queue_run) : P.n;
     };
     // lanes past the last spin stay in the loop (done) for the wave operations
     uint32_t i = QUEUE ? take(true) : block * blockDim.x + threadIdx.x;
@@ -1329,12 +1343,14 @@ static void launch_one(const StepParams &p, uint32_t iters, hipStream_t s) {
         const uint32_t slots = resident_walk_blocks();
         if (p.queue && !p.traj && p.n >= 2 * slots * kWalkBlockSpins && iters >= kQueueMinSteps) {
             if (hipMemsetAsync(p.queue, 0, kXcds * sizeof(uint32_t), s) != hipSuccess) return;
+            StepParams pq = p;
+            pq.queue_run = queue_run(p.n);
             if (p.cell_mt)
                 hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsCellMajor>), dim3(slots),
-                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+                                   dim3(kWalkBlockSpins), 0, s, pq, iters);
             else
                 hipLaunchKernelGGL((k_walk_queue<BOUND, INSTR, MOMENT, kRecsIndirect>), dim3(slots),
-                                   dim3(kWalkBlockSpins), 0, s, p, iters);
+                                   dim3(kWalkBlockSpins), 0, s, pq, iters);
             return;
         }
     }
